@@ -1,0 +1,153 @@
+"""The device line evaluator (warpjq_amd/csrc/jscan.h), compiled for the
+host, against the oracle line by line — the same source the gfx950 kernels
+compile, pinned without a GPU. Also the device number parser against glibc
+strtod (correctly rounded, the contract of Rust's f64::from_str that
+json.rs:73-78 relies on)."""
+import ctypes
+import json
+import os
+import random
+import struct
+
+import pytest
+
+import corpus
+import hostsim
+import oracle
+
+AGG = ("count", "sum", "min", "max", "avg", "group")
+CONF = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "conformance.json"), encoding="utf-8"))
+
+
+def _is_agg(q):
+    return any(q.startswith(a) or ("| " + a) in q for a in AGG)
+
+
+def _expected_status(st):
+    if st["lines_in"] == 0:
+        return 4
+    if st["malformed"]:
+        return 1
+    if st["type_errors"]:
+        return 2
+    return 0
+
+
+@pytest.mark.parametrize("label,data", ids=lambda v: v if isinstance(v, str) else "data", argvalues=[
+    ("15% malformed", corpus.hostile(2, 2500, 0.15)),
+    ("crlf", corpus.hostile(3, 600, 0.05, crlf=True)),
+    ("edge", corpus.edge_corpus()),
+])
+def test_per_line_status_and_rows(label, data):
+    lines = data.split(b"\n")
+    compared = 0
+    for q in corpus.QUERIES:
+        for csv in (False, True):
+            ev = hostsim.Evaluator(q, csv)
+            for ln in lines:
+                try:
+                    out, st = oracle.run(q, ln + b"\n", fmt=oracle.FMT_CSV if csv else oracle.FMT_NDJSON, threads=1)
+                except oracle.RunError:
+                    break  # --csv on a whole-line query
+                o, row, gk = ev.eval(ln)
+                if csv and b"\n" in out:
+                    out = out.split(b"\n", 1)[1]
+                assert o.status == _expected_status(st), (q, csv, ln)
+                if o.status == 0 and not _is_agg(q):
+                    assert (row if o.selected else b"") == out, (q, csv, ln)
+                    compared += 1
+    assert compared > len(lines) // 4
+
+
+def test_aggregate_values_and_group_keys():
+    data = corpus.hostile(9, 1500, 0.05)
+    for q, key in [("group_by(.host) | sum(.bytes)", "host"), ("group_by(.flag) | count", "flag"),
+                   ("group_by(.msg) | count", "msg"), ("group_by(.n) | count", "n")]:
+        ev = hostsim.Evaluator(q)
+        for ln in data.split(b"\n"):
+            o, _, gk = ev.eval(ln)
+            if o.status != 0 or not o.selected:
+                continue
+            out, _ = oracle.run(q, ln + b"\n", threads=1)
+            row = json.loads(out)
+            v = row[key]
+            if o.gkind == 4:  # string: decoded bytes
+                assert gk.decode("utf-8", "surrogatepass") == v or gk == v.encode("utf-8", "surrogatepass")
+            elif o.gkind == 1:
+                assert v is None
+            elif o.gkind == 2:
+                assert v == (gk == b"true")
+            else:
+                assert out.split(b":", 1)[1].rsplit(b",", 1)[0] == gk
+
+
+def test_conformance_verdicts():
+    for name, doc in CONF["must_accept"]:
+        assert hostsim.validate(doc.encode()) == 0, name
+    for name, doc in CONF["must_reject"]:
+        assert hostsim.validate(doc.encode()) == 1, name
+    for name, doc, acc in CONF["implementation_defined"]:
+        assert hostsim.validate(doc.encode()) == (0 if acc else 1), name
+
+
+def test_depth_limit_declines():
+    # 65 brackets hold 64 open containers plus one empty one: still on device
+    assert hostsim.validate(b"[" * 65 + b"]" * 65) == 0
+    assert hostsim.validate(b"[" * 66 + b"]" * 66) == 3  # FALLBACK past the 64-level stack
+    assert hostsim.validate(b"[" * 66) == 3
+
+
+_libc = ctypes.CDLL("libc.so.6")
+_libc.strtod.restype = ctypes.c_double
+_libc.strtod.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
+
+
+def _gen(rng):
+    k = rng.randrange(9)
+    if k == 0:
+        return repr(rng.random() * 10.0 ** rng.randrange(-300, 300))
+    if k == 1:
+        return "%.17g" % rng.uniform(0, 1e6)
+    if k == 2:
+        return str(rng.randrange(0, 2 ** 64))
+    if k == 3:
+        return "%de%d" % (rng.randrange(1, 10 ** rng.randrange(1, 25)), rng.randrange(-350, 320))
+    if k == 4:
+        return "0." + "0" * rng.randrange(0, 30) + str(rng.randrange(1, 10 ** rng.randrange(1, 30)))
+    if k == 5:
+        return str(rng.randrange(1, 10 ** rng.randrange(1, 40))) + "." + str(rng.randrange(0, 10 ** rng.randrange(1, 30)))
+    if k == 6:
+        return "%.*e" % (rng.randrange(0, 25), struct.unpack("<d", struct.pack("<Q", rng.getrandbits(62)))[0])
+    if k == 7:
+        return "-" + str(rng.randrange(0, 10 ** 16))
+    return str(2 ** 53 + rng.randrange(-5, 5))
+
+
+def test_number_parser_matches_strtod():
+    rng = random.Random(11)
+    undecided = 0
+    n = 150000
+    for _ in range(n):
+        s = _gen(rng)
+        if "inf" in s or "nan" in s:
+            continue
+        r = hostsim.parse_number(s.encode())
+        if r is None:
+            undecided += 1
+            continue
+        ref = _libc.strtod(s.encode(), None)
+        assert struct.pack("<d", r[0]) == struct.pack("<d", ref), s
+    assert undecided < n // 1000
+
+
+def test_number_parser_hard_cases():
+    for s in ["9007199254740993", "2.2250738585072011e-308", "2.2250738585072012e-308", "4.9e-324",
+              "2.4703282292062327e-324", "2.4703282292062328e-324", "1.7976931348623157e308",
+              "1.7976931348623158e308", "1e400", "1e-400", "0.1", "-0", "123456789012345678901234567890",
+              "9007199254740992.5", "7.2057594037927933e16", "0.000000000000000000000000000001"]:
+        r = hostsim.parse_number(s.encode())
+        assert r is not None, s
+        assert struct.pack("<d", r[0]) == struct.pack("<d", _libc.strtod(s.encode(), None)), s
+    assert hostsim.parse_number(b"-0")[1] and hostsim.parse_number(b"9007199254740992")[1]
+    assert not hostsim.parse_number(b"9007199254740993")[1]  # past 2^53: not exact as an integer
+    assert not hostsim.parse_number(b"1.0")[1]

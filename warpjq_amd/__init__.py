@@ -1,0 +1,297 @@
+"""warpjq's GPU backend for AMD Instinct MI355X (gfx950).
+
+Python access to libwarpjq_amd.so: the reference's device ABI
+(include/warpjq_kernels.h, the ten entry points warpjq's Rust host binds in
+crates/warpjq-core/src/gpu/ffi.rs:149-184) plus the additive exports of
+include/warpjq_amd.h (the chunked host driver, HBM-resident submit, exact
+aggregate parts, device timing).
+
+Everything here calls the HIP library; there is no CPU path. Loading fails
+loudly when the library has not been built.
+"""
+import ctypes
+import os
+import subprocess
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwarpjq_amd.so")
+
+FMT_NDJSON, FMT_CSV, FMT_COUNT = 0, 1, 2
+WARN, SKIP, ABORT = 0, 1, 2
+GPU_CHUNK = 64 << 20          # gpu/mod.rs:57
+MAX_LINE = 64 << 20           # chunk.rs:22
+NONE = 0xFFFFFFFF
+
+STATUS_NAMES = {0: "ok", 1: "hip error", 2: "no device", 3: "out of memory", 4: "abi mismatch",
+                5: "invalid argument"}
+
+u32, u64, f64 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double
+P = ctypes.POINTER
+
+
+class Step(ctypes.Structure):
+    _fields_ = [("kind", u32), ("index", u32), ("key_off", u32), ("key_len", u32), ("key_hash", u32)]
+
+
+class PathT(ctypes.Structure):
+    _fields_ = [("step_off", u32), ("step_count", u32)]
+
+
+class Cmp(ctypes.Structure):
+    _fields_ = [("path", u32), ("op", u32), ("lit_kind", u32), ("lit_off", u32), ("lit_len", u32),
+                ("_pad", u32), ("lit_num", f64)]
+
+
+class CondOp(ctypes.Structure):
+    _fields_ = [("op", u32), ("arg", u32)]
+
+
+class Program(ctypes.Structure):
+    _fields_ = [("steps", P(Step)), ("n_steps", u32), ("paths", P(PathT)), ("n_paths", u32),
+                ("cmps", P(Cmp)), ("n_cmps", u32), ("cond_rpn", P(CondOp)), ("n_cond", u32),
+                ("blob", P(ctypes.c_uint8)), ("n_blob", u32), ("needed_paths", P(u32)), ("n_needed", u32),
+                ("slot_of_path", P(u32)), ("output_kind", u32), ("output_path", u32), ("agg_kind", u32),
+                ("agg_path", u32), ("group_path", u32), ("has_filter", u32), ("n_fields", u32),
+                ("field_paths", P(u32)), ("prefix_off", P(u32)), ("prefix_len", P(u32)),
+                ("suffix_off", u32), ("suffix_len", u32), ("csv_mode", u32)]
+
+
+class AggPartial(ctypes.Structure):
+    _fields_ = [("count", u64), ("numeric", u64), ("sum", f64), ("min", f64), ("max", f64),
+                ("saw_non_numeric", u32), ("_pad", u32)]
+
+
+class Group(ctypes.Structure):
+    _fields_ = [("key_off", u32), ("key_len", u32), ("key_kind", u32), ("_pad", u32), ("count", u64),
+                ("numeric", u64), ("sum", f64), ("min", f64), ("max", f64)]
+
+
+class ChunkResult(ctypes.Structure):
+    _fields_ = [("n_lines", u64), ("n_blank", u64), ("n_invalid", u64), ("n_type_error", u64),
+                ("n_fallback", u64), ("n_selected", u64), ("out_bytes", P(ctypes.c_uint8)), ("out_len", u64),
+                ("out_line_idx", P(u32)), ("out_row_off", P(u64)), ("fallback_idx", P(u32)),
+                ("fallback_off", P(u32)), ("fallback_len", P(u32)), ("chunk_overflow", u32),
+                ("first_invalid", u32), ("agg", AggPartial), ("groups", P(Group)), ("n_groups", u32),
+                ("group_overflow", u32)]
+
+
+class AggExt(ctypes.Structure):
+    _fields_ = [("isum_lo", u64), ("isum_hi", u64), ("fsum", f64), ("fcomp", f64)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("bytes_in", u64), ("lines_in", u64), ("lines_out", u64), ("malformed", u64),
+                ("type_errors", u64), ("gpu_fallback_lines", u64), ("gpu_redone_chunks", u64),
+                ("h2d_ms", f64), ("kernel_ms", f64), ("chunks", u64), ("err_line", u64)]
+
+
+# The exported C symbols of include/warpjq_kernels.h and include/warpjq_amd.h.
+REFERENCE_SYMBOLS = ["warpjq_probe", "warpjq_abi_check", "warpjq_ctx_create", "warpjq_ctx_destroy",
+                     "warpjq_slot_buffer", "warpjq_slot_capacity", "warpjq_max_lines", "warpjq_submit",
+                     "warpjq_wait", "warpjq_device_name"]
+ADDITIVE_SYMBOLS = ["warpjq_amd_ext_value", "warpjq_amd_wait_ext", "warpjq_amd_submit_device",
+                    "warpjq_amd_slot_timing", "warpjq_amd_kernel_width", "warpjq_amd_run_buffer",
+                    "warpjq_amd_free", "warpjq_amd_compile", "warpjq_amd_program_view",
+                    "warpjq_amd_program_free"]
+
+
+def build(jobs=8):
+    """Compiles libwarpjq_amd.so for gfx950 in-tree (hipcc; no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", os.path.join(_HERE, "csrc"), f"-j{jobs}"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run warpjq_amd.build() (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        cp, sz = ctypes.c_char_p, ctypes.c_size_t
+        L.warpjq_probe.argtypes = [cp, sz]
+        L.warpjq_abi_check.argtypes = [P(u64), sz]
+        L.warpjq_ctx_create.argtypes = [P(Program), u64, u32, P(ctypes.c_void_p), cp, sz]
+        L.warpjq_ctx_destroy.argtypes = [ctypes.c_void_p]
+        L.warpjq_ctx_destroy.restype = None
+        L.warpjq_slot_buffer.argtypes = [ctypes.c_void_p, u32]
+        L.warpjq_slot_buffer.restype = ctypes.c_void_p
+        L.warpjq_slot_capacity.argtypes = [ctypes.c_void_p]
+        L.warpjq_slot_capacity.restype = u64
+        L.warpjq_max_lines.argtypes = [ctypes.c_void_p]
+        L.warpjq_max_lines.restype = u64
+        L.warpjq_submit.argtypes = [ctypes.c_void_p, u32, u64, cp, sz]
+        L.warpjq_wait.argtypes = [ctypes.c_void_p, u32, P(ChunkResult), cp, sz]
+        L.warpjq_device_name.argtypes = [cp, sz]
+        L.warpjq_amd_ext_value.argtypes = [P(AggExt)]
+        L.warpjq_amd_ext_value.restype = f64
+        L.warpjq_amd_wait_ext.argtypes = [ctypes.c_void_p, u32, P(AggExt), P(P(AggExt))]
+        L.warpjq_amd_submit_device.argtypes = [ctypes.c_void_p, u32, ctypes.c_void_p, u64, cp, sz]
+        L.warpjq_amd_slot_timing.argtypes = [ctypes.c_void_p, u32, P(ctypes.c_float), P(ctypes.c_float)]
+        L.warpjq_amd_kernel_width.argtypes = [ctypes.c_void_p]
+        L.warpjq_amd_kernel_width.restype = u32
+        L.warpjq_amd_run_buffer.argtypes = [cp, cp, u64, ctypes.c_int, ctypes.c_int, u64, u64,
+                                            P(P(ctypes.c_uint8)), P(u64), P(Stats), cp, sz]
+        L.warpjq_amd_run_buffer.restype = ctypes.c_int
+        L.warpjq_amd_free.argtypes = [ctypes.c_void_p]
+        L.warpjq_amd_free.restype = None
+        L.warpjq_amd_compile.argtypes = [cp, ctypes.c_int, cp, sz]
+        L.warpjq_amd_compile.restype = ctypes.c_void_p
+        L.warpjq_amd_program_view.argtypes = [ctypes.c_void_p]
+        L.warpjq_amd_program_view.restype = P(Program)
+        L.warpjq_amd_program_free.argtypes = [ctypes.c_void_p]
+        L.warpjq_amd_program_free.restype = None
+        _lib = L
+    return _lib
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+class RunError(RuntimeError):
+    def __init__(self, code, line, msg):
+        super().__init__(msg)
+        self.code = code
+        self.line = line
+
+
+def _err():
+    return ctypes.create_string_buffer(512)
+
+
+def probe():
+    e = _err()
+    rc = lib().warpjq_probe(e, 512)
+    if rc != 0:
+        raise DeviceError(f"{STATUS_NAMES.get(rc, rc)}: {e.value.decode()}")
+
+
+def device_name():
+    b = ctypes.create_string_buffer(256)
+    if lib().warpjq_device_name(b, 256) != 0:
+        return None
+    return b.value.decode()
+
+
+def abi_sizes():
+    return [ctypes.sizeof(t) for t in (Step, PathT, Cmp, CondOp, AggPartial, Group)]
+
+
+def abi_check(sizes=None):
+    s = (u64 * 6)(*(sizes or abi_sizes()))
+    return lib().warpjq_abi_check(s, 6)
+
+
+def run(query, data, fmt=FMT_NDJSON, on_invalid=SKIP, chunk_bytes=GPU_CHUNK, max_line=MAX_LINE):
+    """Runs `query` over `data` on the GPU. Returns (output bytes, stats dict)."""
+    L = lib()
+    out = P(ctypes.c_uint8)()
+    out_len = u64()
+    st = Stats()
+    e = ctypes.create_string_buffer(2048)
+    q = query.encode() if isinstance(query, str) else query
+    rc = L.warpjq_amd_run_buffer(q, data, len(data), fmt, on_invalid, chunk_bytes, max_line, ctypes.byref(out),
+                                 ctypes.byref(out_len), ctypes.byref(st), e, 2048)
+    stats = {k: getattr(st, k) for k, _ in Stats._fields_}
+    if rc != 0:
+        raise RunError(rc, st.err_line, e.value.decode(errors="replace"))
+    b = ctypes.string_at(out, out_len.value) if out_len.value else b""
+    L.warpjq_amd_free(out)
+    return b, stats
+
+
+class Compiled:
+    """A lowered program (warpjq_amd_compile)."""
+
+    def __init__(self, query, csv=False):
+        e = ctypes.create_string_buffer(2048)
+        self.h = lib().warpjq_amd_compile(query.encode(), 1 if csv else 0, e, 2048)
+        if not self.h:
+            raise ValueError(e.value.decode())
+        self.view = lib().warpjq_amd_program_view(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().warpjq_amd_program_free(self.h)
+            self.h = None
+
+
+class Context:
+    """A device context over the raw ABI (warpjq_ctx_create ... destroy)."""
+
+    def __init__(self, query, chunk_bytes=GPU_CHUNK, n_slots=2, csv=False):
+        self.prog = Compiled(query, csv)
+        self.ctx = ctypes.c_void_p()
+        e = _err()
+        rc = lib().warpjq_ctx_create(self.prog.view, chunk_bytes, n_slots, ctypes.byref(self.ctx), e, 512)
+        if rc != 0:
+            raise DeviceError(f"ctx_create: {STATUS_NAMES.get(rc, rc)}: {e.value.decode()}")
+        self.n_slots = n_slots
+
+    def close(self):
+        if self.ctx:
+            lib().warpjq_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def capacity(self):
+        return lib().warpjq_slot_capacity(self.ctx)
+
+    @property
+    def max_lines(self):
+        return lib().warpjq_max_lines(self.ctx)
+
+    def fill(self, slot, data):
+        buf = lib().warpjq_slot_buffer(self.ctx, slot)
+        ctypes.memmove(buf, data, len(data))
+
+    def submit(self, slot, n):
+        e = _err()
+        rc = lib().warpjq_submit(self.ctx, slot, n, e, 512)
+        if rc != 0:
+            raise DeviceError(f"submit: {STATUS_NAMES.get(rc, rc)}: {e.value.decode()}")
+
+    def submit_device(self, slot, dptr, n):
+        e = _err()
+        rc = lib().warpjq_amd_submit_device(self.ctx, slot, dptr, n, e, 512)
+        if rc != 0:
+            raise DeviceError(f"submit_device: {STATUS_NAMES.get(rc, rc)}: {e.value.decode()}")
+
+    def wait(self, slot):
+        r = ChunkResult()
+        e = _err()
+        rc = lib().warpjq_wait(self.ctx, slot, ctypes.byref(r), e, 512)
+        if rc != 0:
+            raise DeviceError(f"wait: {STATUS_NAMES.get(rc, rc)}: {e.value.decode()}")
+        return r
+
+    def ext(self, slot):
+        x = AggExt()
+        g = P(AggExt)()
+        lib().warpjq_amd_wait_ext(self.ctx, slot, ctypes.byref(x), ctypes.byref(g))
+        return x, g
+
+    def timing(self, slot):
+        a, b = ctypes.c_float(), ctypes.c_float()
+        lib().warpjq_amd_slot_timing(self.ctx, slot, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+
+def result_dict(r):
+    """Plain copy of a warpjq_chunk_result's scalar fields and arrays."""
+    d = {k: getattr(r, k) for k in ("n_lines", "n_blank", "n_invalid", "n_type_error", "n_fallback",
+                                    "n_selected", "out_len", "chunk_overflow", "first_invalid", "n_groups",
+                                    "group_overflow")}
+    d["agg"] = {k: getattr(r.agg, k) for k, _ in AggPartial._fields_ if k != "_pad"}
+    d["out"] = ctypes.string_at(r.out_bytes, r.out_len) if r.out_len else b""
+    d["line_idx"] = [r.out_line_idx[i] for i in range(r.n_selected)] if r.n_selected else []
+    d["row_off"] = [r.out_row_off[i] for i in range(r.n_selected + 1)] if r.n_selected else []
+    d["fallback"] = sorted((r.fallback_idx[i], r.fallback_off[i], r.fallback_len[i]) for i in range(r.n_fallback))
+    d["groups"] = [{k: getattr(r.groups[i], k) for k, _ in Group._fields_ if k != "_pad"}
+                   for i in range(r.n_groups)]
+    return d

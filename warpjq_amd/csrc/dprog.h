@@ -1,0 +1,131 @@
+// Host-side digestion of a warpjq_program into the per-slot tables and masks
+// the kernels read (jscan.h DProg). Used at context creation.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "jscan.h"
+
+namespace wjqd {
+
+struct DProgHost {
+  std::vector<SlotPath> spath;
+  std::vector<uint32_t> field_slot;
+  DProg d{};  // host pointers into the caller's warpjq_program and into this
+  uint32_t ns = 1;  // kernel template width: 1, 2, 4, 8 or 16 slots
+};
+
+// Returns false (with a message) for a program the kernels cannot host.
+inline bool build_dprog(const warpjq_program &p, DProgHost &h, std::string &err) {
+  const uint32_t NONE = 0xFFFFFFFFu;
+  if (p.n_needed > 16) {
+    err = "more distinct field paths than the kernel slot table holds";
+    return false;
+  }
+  if (p.output_kind > WARPJQ_OUT_AGG) {
+    err = "unknown output kind";
+    return false;
+  }
+  h.spath.assign(p.n_needed, SlotPath{});
+  uint32_t fused = 0;
+  for (uint32_t s = 0; s < p.n_needed; s++) {
+    uint32_t pid = p.needed_paths[s];
+    if (pid >= p.n_paths) {
+      err = "needed path out of range";
+      return false;
+    }
+    const warpjq_path &pp = p.paths[pid];
+    if (pp.step_count == 0 || (uint64_t)pp.step_off + pp.step_count > p.n_steps) {
+      err = "path steps out of range";
+      return false;
+    }
+    SlotPath sp{pp.step_count, pp.step_off, 0, 0};
+    bool ok = pp.step_count <= WJQ_FUSED_STEPS;
+    for (uint32_t k = 0; k < pp.step_count; k++) {
+      const warpjq_step &st = p.steps[pp.step_off + k];
+      if (st.kind == WARPJQ_STEP_INDEX && st.index >= 0xFFFFu) ok = false;
+      if (st.kind == WARPJQ_STEP_KEY && (uint64_t)st.key_off + st.key_len > p.n_blob) {
+        err = "key out of range";
+        return false;
+      }
+    }
+    sp.fused = ok ? 1 : 0;
+    if (ok) fused |= 1u << s;
+    h.spath[s] = sp;
+  }
+  auto slot_of = [&](uint32_t pid) -> uint32_t {
+    if (pid == NONE || pid >= p.n_paths) return NONE;
+    return p.slot_of_path[pid];
+  };
+  uint32_t filter_mask = 0;
+  for (uint32_t k = 0; k < p.n_cond; k++) {
+    const warpjq_cond_op &o = p.cond_rpn[k];
+    uint32_t s = NONE;
+    if (o.op == WARPJQ_COND_CMP) {
+      if (o.arg >= p.n_cmps) {
+        err = "comparison index out of range";
+        return false;
+      }
+      s = slot_of(p.cmps[o.arg].path);
+      const warpjq_cmp &c = p.cmps[o.arg];
+      if (c.lit_kind == WARPJQ_LIT_STR && (uint64_t)c.lit_off + c.lit_len > p.n_blob) {
+        err = "literal out of range";
+        return false;
+      }
+    } else if (o.op == WARPJQ_COND_TRUTHY) {
+      s = slot_of(o.arg);
+    } else {
+      continue;
+    }
+    if (s == NONE || s >= p.n_needed) {
+      err = "filter path has no slot";
+      return false;
+    }
+    filter_mask |= 1u << s;
+  }
+  uint32_t all = p.n_needed == 32 ? ~0u : ((1u << p.n_needed) - 1);
+  h.field_slot.assign(p.n_fields, 0);
+  for (uint32_t f = 0; f < p.n_fields; f++) {
+    uint32_t s = slot_of(p.field_paths[f]);
+    if (s == NONE || s >= p.n_needed) {
+      err = "projected path has no slot";
+      return false;
+    }
+    h.field_slot[f] = s;
+  }
+  DProg &d = h.d;
+  d.steps = p.steps;
+  d.cmps = p.cmps;
+  d.cond = p.cond_rpn;
+  d.blob = p.blob;
+  d.spath = h.spath.data();
+  d.slot_of_path = p.slot_of_path;
+  d.field_slot = h.field_slot.data();
+  d.prefix_off = p.prefix_off;
+  d.prefix_len = p.prefix_len;
+  d.n_slots = p.n_needed;
+  d.n_cond = p.n_cond;
+  d.fused_mask = fused;
+  d.filter_mask = filter_mask;
+  d.post_mask = all & ~filter_mask;
+  d.has_filter = p.has_filter;
+  d.output_kind = p.output_kind;
+  d.output_slot = p.output_kind == WARPJQ_OUT_PATH ? slot_of(p.output_path) : NONE;
+  if (p.output_kind == WARPJQ_OUT_PATH && d.output_slot == NONE) {
+    err = "output path has no slot";
+    return false;
+  }
+  d.agg_kind = p.agg_kind;
+  d.agg_slot = p.output_kind == WARPJQ_OUT_AGG ? slot_of(p.agg_path) : NONE;
+  d.group_slot = p.output_kind == WARPJQ_OUT_AGG ? slot_of(p.group_path) : NONE;
+  d.n_fields = p.n_fields;
+  d.suffix_off = p.suffix_off;
+  d.suffix_len = p.suffix_len;
+  d.csv_mode = p.csv_mode;
+  uint32_t n = p.n_needed;
+  h.ns = n <= 1 ? 1 : n <= 2 ? 2 : n <= 4 ? 4 : n <= 8 ? 8 : 16;
+  return true;
+}
+
+}  // namespace wjqd

@@ -1,0 +1,630 @@
+// Host driver over the device ABI: the MI355X counterpart of the reference's
+// GPU backend (crates/warpjq-core/src/gpu/mod.rs:244-1259) for in-memory
+// input.
+//
+//   chunking        newline-aligned chunks of at most the slot capacity,
+//                   trimmed to the device line budget (gpu/mod.rs:697-810)
+//   pipeline        two slots: chunk k fills and submits while chunk k-1
+//                   drains (gpu/mod.rs:487-565)
+//   drain / merge   counters -> stats, strict handling, aggregate and group
+//                   merge, rows written in input order (gpu/mod.rs:959-1191)
+//   totals          jq number rendering and sorted groups
+//                   (agg.rs:182-278, output.rs:267-347, exec/mod.rs:321-339)
+//
+// Differences, all in the direction of "the GPU does the work":
+//   * a chunk the device cannot represent (chunk_overflow / group_overflow)
+//     is split in two at a newline and resubmitted instead of being redone
+//     on the CPU;
+//   * sums merge exactly (128-bit integer part + compensated FP64) and round
+//     once at the end;
+//   * there is no CPU evaluator: a line the kernels decline is an error
+//     reported with its line number.
+#include <algorithm>
+#include <charconv>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/warpjq_amd.h"
+#include "../../include/warpjq_kernels.h"
+#include "query.h"
+
+namespace {
+
+enum Format { F_NDJSON = 0, F_CSV = 1, F_COUNT = 2 };
+enum OnInvalid { OI_WARN = 0, OI_SKIP = 1, OI_ABORT = 2 };
+
+// ---- aggregates (agg.rs:15-98) with exact sum parts ----------------------
+
+struct Acc {
+  uint64_t count = 0, numeric = 0;
+  unsigned __int128 isum = 0;  // two's complement
+  double fsum = 0.0, fcomp = 0.0;
+  double min = INFINITY, max = -INFINITY;
+  bool saw_non_numeric = false;
+
+  void add_parts(const warpjq_amd_agg_ext &e) {
+    isum += ((unsigned __int128)e.isum_hi << 64) | e.isum_lo;
+    double t = fsum + e.fsum;
+    double bp = t - fsum;
+    fcomp += (fsum - (t - bp)) + (e.fsum - bp) + e.fcomp;
+    fsum = t;
+  }
+  void merge_minmax(uint64_t num, double mn, double mx) {
+    if (num) {
+      if (mn < min) min = mn;
+      if (mx > max) max = mx;
+    }
+  }
+  double sum() const {
+    warpjq_amd_agg_ext e;
+    e.isum_lo = (uint64_t)isum;
+    e.isum_hi = (uint64_t)(isum >> 64);
+    e.fsum = fsum;
+    e.fcomp = fcomp;
+    return warpjq_amd_ext_value(&e);
+  }
+  bool finish(wjq::AggKind k, double &v) const {
+    switch (k) {
+      case wjq::AggKind::Count: v = (double)count; return true;
+      case wjq::AggKind::Sum: v = sum(); return true;
+      case wjq::AggKind::Min: v = min; return numeric > 0;
+      case wjq::AggKind::Max: v = max; return numeric > 0;
+      case wjq::AggKind::Avg: v = sum() / (double)numeric; return numeric > 0;
+    }
+    return false;
+  }
+};
+
+// GroupKey (agg.rs:106-212): rank 0 null, 1 false, 2 true, 3 number,
+// 4 string, 5 composite; numbers order by value and are identified by their
+// raw text; strings by decoded bytes.
+struct GKey {
+  int rank = 0;
+  std::string bytes;
+  uint64_t bits = 0;
+};
+struct GKeyLess {
+  bool operator()(const GKey &a, const GKey &b) const {
+    if (a.rank != b.rank) return a.rank < b.rank;
+    if (a.rank == 3 && a.bits != b.bits) return a.bits < b.bits;
+    return a.bytes < b.bytes;
+  }
+};
+
+static locale_t c_locale() {
+  static locale_t l = newlocale(LC_NUMERIC_MASK, "C", (locale_t)0);
+  return l;
+}
+
+static uint64_t order_bits(double v) {
+  uint64_t b;
+  memcpy(&b, &v, 8);
+  return (b >> 63) ? ~b : (b ^ (1ull << 63));
+}
+
+static void push_utf8(std::string &s, uint32_t cp) {
+  if (cp < 0x80) {
+    s.push_back((char)cp);
+  } else if (cp < 0x800) {
+    s.push_back((char)(0xC0 | (cp >> 6)));
+    s.push_back((char)(0x80 | (cp & 0x3F)));
+  } else if (cp < 0x10000) {
+    s.push_back((char)(0xE0 | (cp >> 12)));
+    s.push_back((char)(0x80 | ((cp >> 6) & 0x3F)));
+    s.push_back((char)(0x80 | (cp & 0x3F)));
+  } else {
+    s.push_back((char)(0xF0 | (cp >> 18)));
+    s.push_back((char)(0x80 | ((cp >> 12) & 0x3F)));
+    s.push_back((char)(0x80 | ((cp >> 6) & 0x3F)));
+    s.push_back((char)(0x80 | (cp & 0x3F)));
+  }
+}
+
+static uint32_t hex4(const uint8_t *p) {
+  uint32_t v = 0;
+  for (int k = 0; k < 4; k++) {
+    uint8_t c = p[k];
+    v = v * 16 + (c <= '9' ? c - '0' : (c | 0x20) - 'a' + 10);
+  }
+  return v;
+}
+
+// Decodes a validated JSON string body (json.rs:594-654 semantics).
+static std::string decode_body(const uint8_t *raw, size_t n) {
+  std::string out;
+  size_t i = 0;
+  while (i < n) {
+    if (raw[i] != '\\') {
+      out.push_back((char)raw[i++]);
+      continue;
+    }
+    uint8_t e = raw[i + 1];
+    switch (e) {
+      case 'b': out.push_back('\x08'); break;
+      case 'f': out.push_back('\x0c'); break;
+      case 'n': out.push_back('\n'); break;
+      case 'r': out.push_back('\r'); break;
+      case 't': out.push_back('\t'); break;
+      case 'u': {
+        uint32_t cp = hex4(raw + i + 2);
+        i += 6;
+        if (cp >= 0xD800 && cp < 0xDC00 && i + 6 <= n && raw[i] == '\\' && raw[i + 1] == 'u') {
+          uint32_t lo = hex4(raw + i + 2);
+          if (lo >= 0xDC00 && lo < 0xE000) {
+            cp = 0x10000 + ((cp - 0xD800) << 10) + (lo - 0xDC00);
+            i += 6;
+          }
+        }
+        if (cp >= 0xD800 && cp < 0xE000) cp = 0xFFFD;
+        push_utf8(out, cp);
+        continue;
+      }
+      default: out.push_back((char)e);
+    }
+    i += 2;
+  }
+  return out;
+}
+
+// device_group_key (gpu/mod.rs:1198-1221).
+static GKey key_from_device(const uint8_t *data, const warpjq_group &g) {
+  GKey k;
+  const uint8_t *raw = data + g.key_off;
+  switch (g.key_kind) {
+    case 0:
+    case 1: k.rank = 0; break;
+    case 2: k.rank = (g.key_len == 4 && memcmp(raw, "true", 4) == 0) ? 2 : 1; break;
+    case 3: {
+      k.rank = 3;
+      k.bytes.assign((const char *)raw, g.key_len);
+      k.bits = order_bits(strtod_l(k.bytes.c_str(), nullptr, c_locale()));
+      break;
+    }
+    case 4:
+      k.rank = 4;
+      k.bytes = memchr(raw, '\\', g.key_len) ? decode_body(raw, g.key_len)
+                                             : std::string((const char *)raw, g.key_len);
+      break;
+    default:
+      k.rank = 5;
+      k.bytes.assign((const char *)raw, g.key_len);
+  }
+  return k;
+}
+
+// ---- number rendering (agg.rs:253-278; ryu 1.0 layout) ---------------------
+
+static std::string ryu_layout(double v) {
+  char buf[64];
+  auto r = std::to_chars(buf, buf + sizeof buf, v, std::chars_format::scientific);
+  std::string s(buf, r.ptr);
+  bool neg = s[0] == '-';
+  size_t p = neg ? 1 : 0;
+  size_t epos = s.find('e');
+  std::string digits;
+  for (size_t i = p; i < epos; i++)
+    if (s[i] != '.') digits.push_back(s[i]);
+  int exp10 = atoi(s.c_str() + epos + 1);
+  int length = (int)digits.size();
+  int k = exp10 - (length - 1);
+  int kk = length + k;
+  std::string out = neg ? "-" : "";
+  if (k >= 0 && kk <= 16) out += digits + std::string(kk - length, '0') + ".0";
+  else if (kk > 0 && kk <= 16) out += digits.substr(0, kk) + "." + digits.substr(kk);
+  else if (kk > -5 && kk <= 0) out += "0." + std::string(-kk, '0') + digits;
+  else if (length == 1) out += digits + "e" + std::to_string(kk - 1);
+  else out += digits.substr(0, 1) + "." + digits.substr(1) + "e" + std::to_string(kk - 1);
+  return out;
+}
+
+static std::string format_number(double v) {
+  if (std::isnan(v)) return "null";
+  if (std::isinf(v)) return v > 0 ? "1.7976931348623157e+308" : "-1.7976931348623157e+308";
+  if (std::trunc(v) == v && std::fabs(v) < 9007199254740992.0) return std::to_string((long long)v);
+  std::string s = ryu_layout(v);
+  size_t e = s.find('e');
+  if (e != std::string::npos && s[e + 1] != '-') s.insert(e + 1, "+");
+  return s;
+}
+
+// ---- writer (output.rs:87-334) --------------------------------------------
+
+static void csv_cell(std::string &out, const std::string &b) {
+  bool q = b.find_first_of(",\"\n\r") != std::string::npos;
+  if (!q) {
+    out += b;
+    return;
+  }
+  out.push_back('"');
+  for (char c : b) {
+    if (c == '"') out.push_back('"');
+    out.push_back(c);
+  }
+  out.push_back('"');
+}
+
+static std::string group_key_json(const GKey &k) {
+  switch (k.rank) {
+    case 0: return "null";
+    case 1: return "false";
+    case 2: return "true";
+    case 4: {
+      std::string s;
+      wjq::write_json_string(s, (const uint8_t *)k.bytes.data(), k.bytes.size());
+      return s;
+    }
+    default: return k.bytes;
+  }
+}
+
+static std::string group_key_name(const wjq::Program &p) {
+  if (!p.group_by) return "key";
+  const auto &steps = p.path(*p.group_by).steps;
+  if (steps.empty()) return "key";
+  if (steps.back().is_index) return "[" + std::to_string(steps.back().index) + "]";
+  return steps.back().key;
+}
+
+struct Chunk {
+  uint64_t off, len;
+};
+
+struct Run {
+  const wjq::Program &prog;
+  const uint8_t *data;
+  uint64_t n;
+  Format fmt;
+  OnInvalid oi;
+  warpjq_ctx *ctx = nullptr;
+  uint64_t cap = 0, max_lines = 0;
+  std::string out;
+  uint64_t rows = 0;
+  warpjq_amd_stats st{};
+  uint64_t lines_seen = 0;
+  Acc scalar;
+  std::map<GKey, Acc, GKeyLess> groups;
+  std::string err;
+  int rc = 0;
+  uint64_t err_line = 0;
+  bool csv_header_done = false;
+};
+
+static int drain(Run &r, uint32_t slot, const Chunk &c, std::vector<Chunk> &redo) {
+  warpjq_chunk_result res;
+  char e[512] = {0};
+  warpjq_status s = warpjq_wait(r.ctx, slot, &res, e, sizeof e);
+  if (s != WARPJQ_OK) {
+    r.err = std::string("waiting for a chunk: ") + e;
+    return 3;
+  }
+  float h2d = 0, kms = 0;
+  warpjq_amd_slot_timing(r.ctx, slot, &h2d, &kms);
+  r.st.h2d_ms += h2d;
+  r.st.kernel_ms += kms;
+  r.st.chunks++;
+  if (res.chunk_overflow || res.group_overflow) {
+    // Split at the newline nearest the middle and run both halves again.
+    const uint8_t *d = r.data + c.off;
+    uint64_t mid = c.len / 2;
+    const uint8_t *nl = (const uint8_t *)memchr(d + mid, '\n', c.len - mid);
+    uint64_t cut = nl ? (uint64_t)(nl - d) + 1 : c.len;
+    if (cut >= c.len) {
+      nl = (const uint8_t *)memrchr(d, '\n', mid);
+      cut = nl ? (uint64_t)(nl - d) + 1 : 0;
+    }
+    if (cut == 0 || cut >= c.len) {
+      r.err = "a single line exceeds the device's per-chunk capacity";
+      return 3;
+    }
+    r.st.gpu_redone_chunks++;
+    redo.push_back({c.off, cut});
+    redo.push_back({c.off + cut, c.len - cut});
+    return 0;
+  }
+  const uint8_t *data = r.data + c.off;
+  uint64_t first_line = r.lines_seen + 1;
+  r.lines_seen += res.n_lines;
+  r.st.lines_in += res.n_lines - res.n_blank;
+  r.st.malformed += res.n_invalid;
+  r.st.type_errors += res.n_type_error;
+  r.st.gpu_fallback_lines += res.n_fallback;
+  if (res.n_invalid && r.oi == OI_ABORT) {
+    r.err_line = res.first_invalid == 0xFFFFFFFFu ? first_line : first_line + res.first_invalid;
+    r.err = "line is not valid JSON";
+    return 2;
+  }
+  if (res.n_fallback) {
+    uint32_t lo = 0xFFFFFFFFu;
+    for (uint64_t k = 0; k < res.n_fallback; k++) lo = std::min(lo, res.fallback_idx[k]);
+    r.err_line = first_line + lo;
+    r.err = std::to_string(res.n_fallback) + " line(s) could not be decided exactly on the device";
+    return 4;
+  }
+  if (r.prog.is_aggregate()) {
+    if (r.prog.group_by) {
+      const warpjq_amd_agg_ext *gx = nullptr;
+      warpjq_amd_wait_ext(r.ctx, slot, nullptr, &gx);
+      for (uint32_t g = 0; g < res.n_groups; g++) {
+        const warpjq_group &G = res.groups[g];
+        Acc &a = r.groups[key_from_device(data, G)];
+        a.count += G.count;
+        a.numeric += G.numeric;
+        a.add_parts(gx[g]);
+        a.merge_minmax(G.numeric, G.min, G.max);
+      }
+    } else {
+      warpjq_amd_agg_ext x;
+      warpjq_amd_wait_ext(r.ctx, slot, &x, nullptr);
+      r.scalar.count += res.agg.count;
+      r.scalar.numeric += res.agg.numeric;
+      r.scalar.add_parts(x);
+      r.scalar.merge_minmax(res.agg.numeric, res.agg.min, res.agg.max);
+      r.scalar.saw_non_numeric |= res.agg.saw_non_numeric != 0;
+    }
+    return 0;
+  }
+  r.rows += res.n_selected;
+  if (r.fmt != F_COUNT && res.n_selected) r.out.append((const char *)res.out_bytes, res.out_len);
+  return 0;
+}
+
+// Newline-aligned chunks of at most `cap` bytes holding at most `max_lines`
+// lines; a line longer than `cap` is an error (gpu/mod.rs:697-810).
+static bool next_chunk(const uint8_t *d, uint64_t n, uint64_t &pos, uint64_t cap, uint64_t max_lines,
+                       uint64_t max_line, Chunk &c, std::string &err) {
+  if (pos >= n) return false;
+  uint64_t want = std::min<uint64_t>(cap, n - pos);
+  uint64_t end;
+  if (pos + want >= n) {
+    end = n;
+  } else {
+    const uint8_t *nl = (const uint8_t *)memrchr(d + pos, '\n', want);
+    if (!nl) {
+      err = "a line is longer than the device chunk (" + std::to_string(cap) + " bytes)";
+      return false;
+    }
+    end = (uint64_t)(nl - d) + 1;
+  }
+  // line budget: count only when the head of the chunk is dense
+  uint64_t sample = std::min<uint64_t>(1 << 16, end - pos);
+  uint64_t seen = 0;
+  for (uint64_t i = 0; i < sample; i++) seen += d[pos + i] == '\n';
+  uint64_t projected = sample ? seen * (end - pos) / sample : 0;
+  if (projected * 2 >= max_lines) {
+    uint64_t lines = 0;
+    for (uint64_t i = pos; i < end; i++)
+      if (d[i] == '\n' && ++lines == max_lines) {
+        if (i + 1 < end) end = i + 1;
+        break;
+      }
+  }
+  if (max_line < cap) {
+    uint64_t s = pos;
+    for (uint64_t i = pos; i < end; i++)
+      if (d[i] == '\n') {
+        if (i - s > max_line) {
+          err = "a line exceeds the " + std::to_string(max_line) + " byte limit";
+          return false;
+        }
+        s = i + 1;
+      }
+    if (end - s > max_line && (end == n || d[end - 1] != '\n')) {
+      err = "a line exceeds the " + std::to_string(max_line) + " byte limit";
+      return false;
+    }
+  }
+  c = {pos, end - pos};
+  pos = end;
+  return true;
+}
+
+static int submit(Run &r, uint32_t slot, const Chunk &c) {
+  uint8_t *buf = warpjq_slot_buffer(r.ctx, slot);
+  memcpy(buf, r.data + c.off, c.len);
+  char e[512] = {0};
+  if (warpjq_submit(r.ctx, slot, c.len, e, sizeof e) != WARPJQ_OK) {
+    r.err = std::string("submitting a chunk: ") + e;
+    return 3;
+  }
+  r.st.bytes_in += c.len;
+  return 0;
+}
+
+}  // namespace
+
+struct warpjq_amd_program {
+  wjq::Program prog;
+  wjq::Lowered low;
+  warpjq_program view;
+};
+
+extern "C" {
+
+void warpjq_amd_free(void *p) { free(p); }
+
+warpjq_amd_program *warpjq_amd_compile(const char *query, int csv, char *err, size_t err_len) {
+  auto *h = new warpjq_amd_program();
+  wjq::QueryError qe;
+  if (!wjq::parse(query, h->prog, qe)) {
+    if (err && err_len) snprintf(err, err_len, "%s", qe.render(query).c_str());
+    delete h;
+    return nullptr;
+  }
+  wjq::lower(h->prog, csv != 0, h->low);
+  h->view = h->low.view();
+  return h;
+}
+
+const warpjq_program *warpjq_amd_program_view(const warpjq_amd_program *p) { return p ? &p->view : nullptr; }
+
+void warpjq_amd_program_free(warpjq_amd_program *p) { delete p; }
+
+int warpjq_amd_run_buffer(const char *query, const uint8_t *data, uint64_t n, int format, int on_invalid,
+                          uint64_t chunk_bytes, uint64_t max_line_bytes, uint8_t **out, uint64_t *out_len,
+                          warpjq_amd_stats *stats, char *err, size_t err_len) {
+  *out = nullptr;
+  *out_len = 0;
+  memset(stats, 0, sizeof *stats);
+  wjq::Program prog;
+  wjq::QueryError qe;
+  if (!wjq::parse(query, prog, qe)) {
+    snprintf(err, err_len, "%s", qe.render(query).c_str());
+    return 1;
+  }
+  if (format == F_CSV && prog.output.kind == wjq::Output::PASSTHROUGH) {  // exec/mod.rs:256-266
+    snprintf(err, err_len, "--csv needs a query with named columns, but this one emits whole lines");
+    return 3;
+  }
+  wjq::Lowered low;
+  wjq::lower(prog, format == F_CSV, low);
+  warpjq_program view = low.view();
+  Run r{prog, data, n, (Format)format, (OnInvalid)on_invalid};
+  // BOM (chunk.rs:210-212)
+  uint64_t pos = 0;
+  if (n >= 3 && data[0] == 0xEF && data[1] == 0xBB && data[2] == 0xBF) pos = 3;
+  r.cap = std::max<uint64_t>(1, std::min<uint64_t>(chunk_bytes, 64ull << 20));
+  if (r.cap < (1u << 20)) r.cap = std::min<uint64_t>(chunk_bytes, r.cap);
+  char e[512] = {0};
+  if (warpjq_ctx_create(&view, r.cap, 2, &r.ctx, e, sizeof e) != WARPJQ_OK) {
+    snprintf(err, err_len, "creating the device context: %s", e);
+    return 3;
+  }
+  r.max_lines = warpjq_max_lines(r.ctx);
+
+  if (r.fmt == F_CSV) {  // write_header
+    if (auto cols = prog.csv_headers()) {
+      for (size_t i = 0; i < cols->size(); i++) {
+        if (i) r.out.push_back(',');
+        csv_cell(r.out, (*cols)[i]);
+      }
+      r.out.push_back('\n');
+    }
+  }
+
+  std::vector<Chunk> pending;  // chunks to resubmit after an overflow, in order
+  bool have_prev = false;
+  Chunk prev{};
+  uint32_t slot = 0;
+  int rc = 0;
+  std::string cerr;
+  for (;;) {
+    Chunk c;
+    bool got;
+    if (!pending.empty()) {
+      c = pending.front();
+      pending.erase(pending.begin());
+      got = true;
+    } else {
+      got = next_chunk(data, n, pos, r.cap, r.max_lines, max_line_bytes, c, cerr);
+      if (!got && !cerr.empty()) {
+        rc = 3;
+        r.err = cerr;
+        break;
+      }
+    }
+    if (!got) break;
+    if (have_prev && !pending.empty()) {
+      // only reached after an overflow redo was queued; keep order strict
+    }
+    if ((rc = submit(r, slot, c))) break;
+    if (have_prev) {
+      std::vector<Chunk> redo;
+      if ((rc = drain(r, slot ^ 1, prev, redo))) break;
+      if (!redo.empty()) {
+        // the previous chunk must be redone before this one: drain this one
+        // too, discard it, and queue redo + this chunk again
+        warpjq_chunk_result tmp;
+        char e2[512];
+        warpjq_wait(r.ctx, slot, &tmp, e2, sizeof e2);
+        r.st.bytes_in -= prev.len + c.len;
+        redo.push_back(c);
+        pending.insert(pending.begin(), redo.begin(), redo.end());
+        have_prev = false;
+        continue;
+      }
+    }
+    prev = c;
+    have_prev = true;
+    slot ^= 1;
+  }
+  if (!rc && have_prev) {
+    std::vector<Chunk> redo;
+    rc = drain(r, slot ^ 1, prev, redo);
+    while (!rc && !redo.empty()) {
+      r.st.bytes_in -= prev.len;
+      std::vector<Chunk> todo = redo;
+      redo.clear();
+      for (size_t i = 0; i < todo.size() && !rc; i++) {
+        std::vector<Chunk> more;
+        if ((rc = submit(r, 0, todo[i]))) break;
+        rc = drain(r, 0, todo[i], more);
+        if (!rc && !more.empty()) {
+          r.st.bytes_in -= todo[i].len;
+          todo.insert(todo.begin() + i + 1, more.begin(), more.end());
+        }
+      }
+    }
+  }
+  warpjq_ctx_destroy(r.ctx);
+  stats->bytes_in = r.st.bytes_in;
+  stats->lines_in = r.st.lines_in;
+  stats->malformed = r.st.malformed;
+  stats->type_errors = r.st.type_errors;
+  stats->gpu_fallback_lines = r.st.gpu_fallback_lines;
+  stats->gpu_redone_chunks = r.st.gpu_redone_chunks;
+  stats->h2d_ms = r.st.h2d_ms;
+  stats->kernel_ms = r.st.kernel_ms;
+  stats->chunks = r.st.chunks;
+  stats->err_line = r.err_line;
+  if (rc) {
+    snprintf(err, err_len, "%s", r.err.c_str());
+    return rc;
+  }
+  // finish_totals (exec/mod.rs:321-339) and Writer::finish
+  if (prog.is_aggregate()) {
+    wjq::AggKind kind = prog.output.agg;
+    if (!prog.group_by) {
+      double v;
+      std::string value = r.scalar.finish(kind, v) ? format_number(v) : "null";
+      if (r.fmt == F_COUNT) {
+        r.out = value + "\n";
+        r.rows = 0;
+        r.fmt = F_NDJSON;
+      } else {
+        r.out += value + "\n";
+        r.rows++;
+      }
+    } else {
+      std::string name = group_key_name(prog);
+      for (auto &kv : r.groups) {
+        r.rows++;
+        double v;
+        std::string value = kv.second.finish(kind, v) ? format_number(v) : "null";
+        if (r.fmt == F_CSV) {
+          csv_cell(r.out, kv.first.rank == 4 ? kv.first.bytes : group_key_json(kv.first));
+          r.out += "," + value;
+        } else {
+          r.out.push_back('{');
+          wjq::write_json_string(r.out, (const uint8_t *)name.data(), name.size());
+          r.out += ":" + group_key_json(kv.first) + ",";
+          const char *kn = wjq::agg_name(kind);
+          wjq::write_json_string(r.out, (const uint8_t *)kn, strlen(kn));
+          r.out += ":" + value + "}";
+        }
+        r.out.push_back('\n');
+      }
+    }
+  }
+  if (r.fmt == F_COUNT) r.out = std::to_string(r.rows) + "\n";
+  stats->lines_out = r.rows;
+  *out = (uint8_t *)malloc(r.out.size() + 1);
+  memcpy(*out, r.out.data(), r.out.size());
+  *out_len = r.out.size();
+  return 0;
+}
+
+}  // extern "C"

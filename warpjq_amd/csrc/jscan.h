@@ -1,0 +1,1210 @@
+// Per-line JSON evaluation shared by the gfx950 kernels (compiled by hipcc
+// with __device__) and by a host build used only by the test-suite to pin the
+// exact same source against the oracle on the CPU.
+//
+// Semantics restated from the reference's CPU engine, which is the oracle the
+// GPU must match byte for byte:
+//   validation          crates/warpjq-core/src/json.rs:213-421
+//   lookup              crates/warpjq-core/src/json.rs:444-591
+//   decoded strings     crates/warpjq-core/src/json.rs:594-654
+//   compare / truthy    crates/warpjq-core/src/json.rs:63-65, :662-693
+//   line outcome order  crates/warpjq-core/src/exec/cpu.rs:299-371
+//   blank lines         crates/warpjq-core/src/chunk.rs:432-438
+//   CSV cells           crates/warpjq-core/src/output.rs:45-81
+//
+// Design: one lane walks one line once. The walk validates the whole value
+// and, in the same pass, captures every needed path of up to WJQ_FUSED_STEPS
+// steps ("fused" paths), tracking for each open container level which paths
+// are still live (a 16-bit mask per level). Duplicate keys overwrite earlier
+// captures, which is jq's last-wins rule; an intermediate value of the wrong
+// kind records a type error; null records Missing. Longer paths are resolved
+// afterwards by a sequential lookup over the already-validated line.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/warpjq_kernels.h"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define WJQ_HD __host__ __device__ __forceinline__
+#define WJQ_HDI __host__ __device__
+#else
+#define WJQ_HD inline
+#define WJQ_HDI inline
+#endif
+
+namespace wjqd {
+
+enum { JK_MISSING = 0, JK_NULL = 1, JK_BOOL = 2, JK_NUM = 3, JK_STR = 4, JK_ARR = 5, JK_OBJ = 6 };
+enum { ST_OK = 0, ST_INVALID = 1, ST_TYPE_ERROR = 2, ST_FALLBACK = 3, ST_BLANK = 4 };
+
+// Paths of at most this many steps are captured during the validating walk.
+#define WJQ_FUSED_STEPS 3
+// Depth of the container bit-stack; deeper lines are declined (FALLBACK).
+#define WJQ_MAX_DEPTH 64
+#define WJQ_END 0x0Au  // '\n', the line terminator; also returned past the chunk
+
+// A captured value. `off` is chunk-relative; Missing is kind 0.
+struct SlotV {
+  uint32_t off;
+  uint32_t len;
+  uint32_t kind;
+};
+
+// Per needed slot: the path it resolves, pre-digested at context creation.
+struct SlotPath {
+  uint32_t nsteps;
+  uint32_t step_off;  // into steps[]
+  uint32_t fused;     // 1 when nsteps <= WJQ_FUSED_STEPS and indices < 0xFFFF
+  uint32_t _pad;
+};
+
+// The device view of a lowered program (warpjq_program plus derived masks).
+struct DProg {
+  const warpjq_step *steps;
+  const warpjq_cmp *cmps;
+  const warpjq_cond_op *cond;
+  const uint8_t *blob;
+  const SlotPath *spath;       // [n_slots]
+  const uint32_t *slot_of_path;
+  const uint32_t *field_slot;  // [n_fields] slot of each projected field
+  const uint32_t *prefix_off;
+  const uint32_t *prefix_len;
+  uint32_t n_slots;
+  uint32_t n_cond;
+  uint32_t fused_mask;   // slots captured by the walk
+  uint32_t filter_mask;  // slots the filter reads
+  uint32_t post_mask;    // every other needed slot
+  uint32_t has_filter;
+  uint32_t output_kind;
+  uint32_t output_slot;  // OUT_PATH
+  uint32_t agg_kind;
+  uint32_t agg_slot;     // 0xFFFFFFFF for count
+  uint32_t group_slot;   // 0xFFFFFFFF when ungrouped
+  uint32_t n_fields;
+  uint32_t suffix_off;
+  uint32_t suffix_len;
+  uint32_t csv_mode;
+};
+
+// ---------------------------------------------------------------------------
+// byte classes
+// ---------------------------------------------------------------------------
+
+WJQ_HD bool is_digit(uint32_t c) { return c - '0' < 10u; }
+WJQ_HD bool is_hexd(uint32_t c) { return c - '0' < 10u || (c | 0x20u) - 'a' < 6u; }
+WJQ_HD uint32_t hexval(uint32_t c) { return c <= '9' ? c - '0' : (c | 0x20u) - 'a' + 10; }
+WJQ_HD bool is_jws(uint32_t c) { return c == ' ' || c == '\t' || c == '\r'; }
+
+// Bit 7 of each byte set when that byte may be '"', '\\' or < 0x20. Exact
+// for the lowest flagged byte; bytes above a true hit can be flagged falsely
+// (borrow), which callers tolerate by re-checking the byte.
+WJQ_HD uint32_t str_special(uint32_t x) {
+  uint32_t q = x ^ 0x22222222u, b = x ^ 0x5C5C5C5Cu;
+  uint32_t m = ((q - 0x01010101u) & ~q) | ((b - 0x01010101u) & ~b) | ((x - 0x20202020u) & ~x);
+  return m & 0x80808080u;
+}
+
+WJQ_HD int ctz32(uint32_t x) { return __builtin_ctz(x); }
+WJQ_HD int clz64(uint64_t x) { return __builtin_clzll(x); }
+
+// ---------------------------------------------------------------------------
+// scanners. `R` provides byte(i) and dword(k) (bytes 4k..4k+3, little
+// endian); both return WJQ_END for positions at or past the chunk end.
+// ---------------------------------------------------------------------------
+
+template <class R>
+WJQ_HD uint32_t skip_ws(R &rd, uint32_t i) {
+  while (is_jws(rd.byte(i))) i++;
+  return i;
+}
+
+// `i` is the opening quote. On success returns true with `i` one past the
+// closing quote; `esc` is set when the body holds a backslash.
+template <class R>
+WJQ_HD bool scan_string(R &rd, uint32_t &i, bool &esc) {
+  uint32_t j = i + 1;
+  for (;;) {
+    uint32_t w = rd.dword(j >> 2);
+    uint32_t m = str_special(w) & ~((1u << ((j & 3) * 8)) - 1u);
+    if (m == 0) {
+      j = (j & ~3u) + 4;
+      continue;
+    }
+    uint32_t k = (uint32_t)ctz32(m) >> 3;
+    uint32_t pos = (j & ~3u) + k;
+    uint32_t c = (w >> (k * 8)) & 0xFF;
+    if (c == '"') {
+      i = pos + 1;
+      return true;
+    }
+    if (c == '\\') {
+      uint32_t e = rd.byte(pos + 1);
+      if (e == '"' || e == '\\' || e == '/' || e == 'b' || e == 'f' || e == 'n' || e == 'r' || e == 't') {
+        j = pos + 2;
+      } else if (e == 'u') {
+        if (!is_hexd(rd.byte(pos + 2)) || !is_hexd(rd.byte(pos + 3)) || !is_hexd(rd.byte(pos + 4)) ||
+            !is_hexd(rd.byte(pos + 5)))
+          return false;
+        j = pos + 6;
+      } else {
+        return false;
+      }
+      esc = true;
+      continue;
+    }
+    if (c < 0x20) return false;  // raw control, or the line ended inside
+    j = pos + 1;                  // false positive of str_special
+  }
+}
+
+// json.rs:370-410. `i` is '-' or a digit; returns one past the number.
+template <class R>
+WJQ_HD bool scan_number(R &rd, uint32_t &i) {
+  uint32_t j = i;
+  if (rd.byte(j) == '-') j++;
+  uint32_t is = j;
+  uint32_t first = rd.byte(j);
+  while (is_digit(rd.byte(j))) j++;
+  if (j == is) return false;
+  if (first == '0' && j - is > 1) return false;
+  if (rd.byte(j) == '.') {
+    j++;
+    uint32_t fs = j;
+    while (is_digit(rd.byte(j))) j++;
+    if (j == fs) return false;
+  }
+  uint32_t c = rd.byte(j);
+  if (c == 'e' || c == 'E') {
+    j++;
+    c = rd.byte(j);
+    if (c == '+' || c == '-') j++;
+    uint32_t es = j;
+    while (is_digit(rd.byte(j))) j++;
+    if (j == es) return false;
+  }
+  i = j;
+  return true;
+}
+
+template <class R>
+WJQ_HD bool scan_lit(R &rd, uint32_t &i, uint32_t c0) {
+  if (c0 == 't') {
+    if (rd.byte(i + 1) != 'r' || rd.byte(i + 2) != 'u' || rd.byte(i + 3) != 'e') return false;
+    i += 4;
+  } else if (c0 == 'f') {
+    if (rd.byte(i + 1) != 'a' || rd.byte(i + 2) != 'l' || rd.byte(i + 3) != 's' || rd.byte(i + 4) != 'e')
+      return false;
+    i += 5;
+  } else {
+    if (rd.byte(i + 1) != 'u' || rd.byte(i + 2) != 'l' || rd.byte(i + 3) != 'l') return false;
+    i += 4;
+  }
+  return true;
+}
+
+WJQ_HD uint32_t kind_of(uint32_t c) {
+  switch (c) {
+    case '{': return JK_OBJ;
+    case '[': return JK_ARR;
+    case '"': return JK_STR;
+    case 't':
+    case 'f': return JK_BOOL;
+    case 'n': return JK_NULL;
+    default: return JK_NUM;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// decoded string iteration (json.rs:594-654): escapes decoded, surrogate
+// pairs combined, lone surrogates replaced by U+FFFD, all as UTF-8 bytes.
+// ---------------------------------------------------------------------------
+
+template <class R>
+struct DecIter {
+  R *rd;
+  uint32_t p, end;
+  uint32_t pend;  // pending UTF-8 bytes, low byte first
+  uint32_t npend;
+
+  WJQ_HD DecIter(R &r, uint32_t a, uint32_t b) : rd(&r), p(a), end(b), pend(0), npend(0) {}
+
+  WJQ_HD uint32_t hex4(uint32_t q) {
+    return (hexval(rd->byte(q)) << 12) | (hexval(rd->byte(q + 1)) << 8) | (hexval(rd->byte(q + 2)) << 4) |
+           hexval(rd->byte(q + 3));
+  }
+
+  // Next decoded byte, or -1 at the end. Assumes a validated body.
+  WJQ_HD int next() {
+    if (npend) {
+      uint32_t b = pend & 0xFF;
+      pend >>= 8;
+      npend--;
+      return (int)b;
+    }
+    if (p >= end) return -1;
+    uint32_t c = rd->byte(p);
+    if (c != '\\') {
+      p++;
+      return (int)c;
+    }
+    uint32_t e = rd->byte(p + 1);
+    p += 2;
+    switch (e) {
+      case 'b': return 0x08;
+      case 'f': return 0x0C;
+      case 'n': return '\n';
+      case 'r': return '\r';
+      case 't': return '\t';
+      case 'u': break;
+      default: return (int)e;  // " \ /
+    }
+    uint32_t cp = hex4(p);
+    p += 4;
+    if (cp >= 0xD800 && cp < 0xDC00 && p + 6 <= end && rd->byte(p) == '\\' && rd->byte(p + 1) == 'u') {
+      uint32_t lo = hex4(p + 2);
+      if (lo >= 0xDC00 && lo < 0xE000) {
+        cp = 0x10000 + ((cp - 0xD800) << 10) + (lo - 0xDC00);
+        p += 6;
+      }
+    }
+    if (cp >= 0xD800 && cp < 0xE000) cp = 0xFFFD;
+    if (cp < 0x80) return (int)cp;
+    uint32_t b0;
+    if (cp < 0x800) {
+      b0 = 0xC0 | (cp >> 6);
+      pend = 0x80 | (cp & 0x3F);
+      npend = 1;
+    } else if (cp < 0x10000) {
+      b0 = 0xE0 | (cp >> 12);
+      pend = (0x80 | ((cp >> 6) & 0x3F)) | ((0x80 | (cp & 0x3F)) << 8);
+      npend = 2;
+    } else {
+      b0 = 0xF0 | (cp >> 18);
+      pend = (0x80 | ((cp >> 12) & 0x3F)) | ((0x80 | ((cp >> 6) & 0x3F)) << 8) | ((0x80 | (cp & 0x3F)) << 16);
+      npend = 3;
+    }
+    return (int)b0;
+  }
+};
+
+// FNV-1a 32 over decoded bytes of the body [a, b) (query/mod.rs:15-22).
+template <class R>
+WJQ_HD uint32_t decoded_hash(R &rd, uint32_t a, uint32_t b, bool esc, uint32_t *dlen) {
+  uint32_t h = 0x811c9dc5u, n = 0;
+  if (!esc) {
+    for (uint32_t q = a; q < b; q++) {
+      h = (h ^ rd.byte(q)) * 0x01000193u;
+    }
+    n = b - a;
+  } else {
+    DecIter<R> it(rd, a, b);
+    for (int c; (c = it.next()) >= 0; n++) h = (h ^ (uint32_t)c) * 0x01000193u;
+  }
+  if (dlen) *dlen = n;
+  return h;
+}
+
+// Three-way compare of the decoded body [a, b) with bytes lit[0, n).
+template <class R>
+WJQ_HD int decoded_cmp(R &rd, uint32_t a, uint32_t b, bool esc, const uint8_t *lit, uint32_t n) {
+  if (!esc) {
+    uint32_t m = b - a;
+    uint32_t k = m < n ? m : n;
+    for (uint32_t q = 0; q < k; q++) {
+      uint32_t x = rd.byte(a + q), y = lit[q];
+      if (x != y) return x < y ? -1 : 1;
+    }
+    return m < n ? -1 : (m > n ? 1 : 0);
+  }
+  DecIter<R> it(rd, a, b);
+  uint32_t q = 0;
+  for (;;) {
+    int c = it.next();
+    if (c < 0) return q < n ? -1 : 0;
+    if (q >= n) return 1;
+    uint32_t y = lit[q++];
+    if ((uint32_t)c != y) return (uint32_t)c < y ? -1 : 1;
+  }
+}
+
+// Does the string body [a, b) contain a backslash?
+template <class R>
+WJQ_HD bool body_has_escape(R &rd, uint32_t a, uint32_t b) {
+  for (uint32_t q = a; q < b; q++)
+    if (rd.byte(q) == '\\') return true;
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// number value (Slot::as_f64, json.rs:73-78): correctly rounded decimal ->
+// binary64. Clinger's exact fast path, then Eisel-Lemire with a 128-bit
+// power-of-five table (Lemire 2021), then for inputs with more than 19
+// significant digits the two-candidate check. Anything still ambiguous is
+// reported as not decided (the caller declines the line).
+// ---------------------------------------------------------------------------
+
+#ifndef WJQ_TABLE_QUAL
+#if defined(__HIPCC__)
+#define WJQ_TABLE_QUAL __device__ __constant__ static const
+#else
+#define WJQ_TABLE_QUAL static const
+#endif
+#endif
+#include "pow5_table.h"
+
+WJQ_HD void mul128(uint64_t a, uint64_t b, uint64_t &hi, uint64_t &lo) {
+  unsigned __int128 p = (unsigned __int128)a * b;
+  hi = (uint64_t)(p >> 64);
+  lo = (uint64_t)p;
+}
+
+// w * 10^q for w != 0 as binary64 bits; false when the product is too close
+// to a rounding boundary to decide from 128 bits.
+WJQ_HD bool eisel_lemire(uint64_t w, int q, uint64_t &bits) {
+  if (q < WJQ_POW5_MIN_Q) {
+    bits = 0;
+    return true;
+  }
+  if (q > WJQ_POW5_MAX_Q) {
+    bits = 0x7FF0000000000000ull;
+    return true;
+  }
+  int lz = clz64(w);
+  w <<= lz;
+  const uint64_t *t = &wjq_pow5_128[2 * (q - WJQ_POW5_MIN_Q)];
+  uint64_t hi, lo;
+  mul128(w, t[0], hi, lo);
+  if ((hi & 0x1FF) == 0x1FF) {
+    uint64_t hi2, lo2;
+    mul128(w, t[1], hi2, lo2);
+    lo += hi2;
+    if (lo < hi2) hi++;
+  }
+  if (lo == ~0ull && (q < -27 || q > 55)) return false;
+  uint64_t upper = hi >> 63;
+  uint64_t m = hi >> (upper + 9);
+  int p2 = (int)(((217706 * (int64_t)q) >> 16) + 63 + (int64_t)upper - lz + 1023);
+  if (p2 <= 0) {
+    if (-p2 + 1 >= 64) {
+      bits = 0;
+      return true;
+    }
+    m >>= -p2 + 1;
+    m += m & 1;
+    m >>= 1;
+    bits = m;  // subnormal, or the smallest normal when rounding carried
+    return true;
+  }
+  if (lo <= 1 && q >= -4 && q <= 23 && (m & 3) == 1) {
+    if ((m << (upper + 9)) == hi) m &= ~1ull;
+  }
+  m += m & 1;
+  m >>= 1;
+  if (m >= (2ull << 52)) {
+    m = 1ull << 52;
+    p2++;
+  }
+  m &= ~(1ull << 52);
+  if (p2 >= 0x7FF) {
+    bits = 0x7FF0000000000000ull;
+    return true;
+  }
+  bits = ((uint64_t)p2 << 52) | m;
+  return true;
+}
+
+WJQ_HD double bits_to_f64(uint64_t b) {
+  union {
+    uint64_t u;
+    double d;
+  } x;
+  x.u = b;
+  return x.d;
+}
+WJQ_HD uint64_t f64_to_bits(double d) {
+  union {
+    uint64_t u;
+    double d;
+  } x;
+  x.d = d;
+  return x.u;
+}
+
+struct NumVal {
+  double v;
+  int64_t iv;    // exact integer value when is_int
+  bool is_int;   // integer token with |value| <= 2^53
+};
+
+// Token [a, b) is a validated JSON number. Returns false when undecided.
+template <class R>
+WJQ_HD bool parse_number(R &rd, uint32_t a, uint32_t b, NumVal &out) {
+  uint32_t i = a;
+  bool neg = rd.byte(i) == '-';
+  if (neg) i++;
+  uint64_t w = 0;
+  int nd = 0;          // significant digits taken into w
+  int dropped = 0;     // significant digits beyond 19 (integer part)
+  int frac_taken = 0;  // fraction digits consumed into w
+  bool trunc = false;
+  bool any_frac = false, any_exp = false;
+  // integer part
+  for (; i < b; i++) {
+    uint32_t c = rd.byte(i);
+    if (!is_digit(c)) break;
+    if (nd == 0 && c == '0') continue;
+    if (nd < 19) {
+      w = w * 10 + (c - '0');
+      nd++;
+    } else {
+      dropped++;
+      if (c != '0') trunc = true;
+    }
+  }
+  if (i < b && rd.byte(i) == '.') {
+    any_frac = true;
+    i++;
+    for (; i < b; i++) {
+      uint32_t c = rd.byte(i);
+      if (!is_digit(c)) break;
+      if (nd == 0 && c == '0') {
+        frac_taken++;
+        continue;
+      }
+      if (nd < 19) {
+        w = w * 10 + (c - '0');
+        nd++;
+        frac_taken++;
+      } else if (c != '0') {
+        trunc = true;
+      }
+    }
+  }
+  int64_t e = 0;
+  if (i < b && (rd.byte(i) == 'e' || rd.byte(i) == 'E')) {
+    any_exp = true;
+    i++;
+    bool eneg = false;
+    if (rd.byte(i) == '+') i++;
+    else if (rd.byte(i) == '-') {
+      eneg = true;
+      i++;
+    }
+    for (; i < b; i++) {
+      uint32_t c = rd.byte(i);
+      if (e < 100000) e = e * 10 + (c - '0');
+    }
+    if (eneg) e = -e;
+  }
+  int64_t q64 = e + dropped - frac_taken;
+  if (q64 < -100000) q64 = -100000;
+  if (q64 > 100000) q64 = 100000;
+  int q = (int)q64;
+  out.is_int = false;
+  out.iv = 0;
+  if (w == 0) {
+    out.v = neg ? -0.0 : 0.0;
+    if (!any_frac && !any_exp) {
+      out.is_int = true;
+      out.iv = 0;
+    }
+    return true;
+  }
+  if (!any_frac && !any_exp && !trunc && dropped == 0 && w <= (1ull << 53)) {
+    out.is_int = true;
+    out.iv = neg ? -(int64_t)w : (int64_t)w;
+  }
+  if (!trunc && q >= -22 && q <= 22 && w <= (1ull << 53)) {
+    const double p10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,  1e8,  1e9,  1e10, 1e11,
+                            1e12, 1e13, 1e14, 1e15, 1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+    double d = (double)w;
+    d = q < 0 ? d / p10[-q] : d * p10[q];
+    out.v = neg ? -d : d;
+    return true;
+  }
+  uint64_t bits;
+  if (!eisel_lemire(w, q, bits)) return false;
+  if (trunc) {
+    uint64_t bits2;
+    if (!eisel_lemire(w + 1, q, bits2) || bits2 != bits) return false;
+  }
+  double d = bits_to_f64(bits);
+  out.v = neg ? -d : d;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// the walk
+// ---------------------------------------------------------------------------
+
+WJQ_HD uint32_t lvl_get(uint64_t packed, uint32_t L) { return (uint32_t)(packed >> (16 * L)) & 0xFFFFu; }
+WJQ_HD uint64_t lvl_set(uint64_t packed, uint32_t L, uint32_t v) {
+  uint64_t sh = 16ull * L;
+  return (packed & ~(0xFFFFull << sh)) | ((uint64_t)(v & 0xFFFFu) << sh);
+}
+
+// Does the step `st` name the object member whose raw key body is [a, b)?
+template <class R>
+WJQ_HD bool key_step_match(R &rd, const DProg &P, const warpjq_step &st, uint32_t a, uint32_t b, bool esc,
+                           uint32_t h, uint32_t dlen) {
+  if (st.kind != WARPJQ_STEP_KEY || st.key_len != dlen || st.key_hash != h) return false;
+  return decoded_cmp(rd, a, b, esc, P.blob + st.key_off, st.key_len) == 0;
+}
+
+// Validates the value starting at `i` and the whitespace after it up to the
+// end of the line; captures fused slots. Returns ST_OK / ST_INVALID /
+// ST_FALLBACK. On ST_OK `end` is the line terminator's position.
+template <int NS, class R>
+WJQ_HD int walk(R &rd, uint32_t i, const DProg &P, SlotV (&sl)[NS], uint32_t &te, uint32_t &end) {
+  uint64_t arr = 0;  // bit k: the container at depth k is an array
+  uint32_t d = 0;    // open containers
+  uint64_t act = 0;  // live slots per container level 0..3
+  uint64_t fin = 0;  // slots whose captured value is the container at that level
+  uint64_t cnt = 0;  // element index per array level 0..3
+  uint32_t hit = P.fused_mask;  // slots for which the next value matters
+  uint32_t hit_fin = 0;         // ... as the path's final value
+  enum { VALUE, KEY, AFTER } state = VALUE;
+  for (;;) {
+    if (state == VALUE) {
+      i = skip_ws(rd, i);
+      uint32_t c = rd.byte(i);
+      uint32_t kind;
+      if (c == '{') kind = JK_OBJ;
+      else if (c == '[') kind = JK_ARR;
+      else if (c == '"') kind = JK_STR;
+      else if (c == 't' || c == 'f') kind = JK_BOOL;
+      else if (c == 'n') kind = JK_NULL;
+      else if (c == '-' || is_digit(c)) kind = JK_NUM;
+      else return ST_INVALID;
+      uint32_t next_act = 0;
+      if (hit) {
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+          if (!((hit >> s) & 1)) continue;
+          if ((hit_fin >> s) & 1) {
+            sl[s].off = i;
+            sl[s].kind = kind;
+            sl[s].len = 0;
+            te &= ~(1u << s);
+          } else {
+            // intermediate value: the next step of this slot is step d
+            const warpjq_step &st = P.steps[P.spath[s].step_off + d];
+            uint32_t want = st.kind == WARPJQ_STEP_INDEX ? (uint32_t)JK_ARR : (uint32_t)JK_OBJ;
+            sl[s].kind = JK_MISSING;
+            if (kind == JK_NULL) {
+              te &= ~(1u << s);
+            } else if (kind == want) {
+              te &= ~(1u << s);
+              next_act |= 1u << s;
+            } else {
+              te |= 1u << s;
+            }
+          }
+        }
+      }
+      if (kind == JK_OBJ || kind == JK_ARR) {
+        uint32_t close = kind == JK_OBJ ? '}' : ']';
+        uint32_t j = skip_ws(rd, i + 1);
+        if (rd.byte(j) == close) {
+          i = j + 1;
+          if (hit_fin) {
+#pragma unroll
+            for (int s = 0; s < NS; s++)
+              if ((hit_fin >> s) & 1) sl[s].len = i - sl[s].off;
+          }
+          state = AFTER;
+          continue;
+        }
+        if (d >= WJQ_MAX_DEPTH) return ST_FALLBACK;
+        if (kind == JK_ARR) arr |= 1ull << d;
+        else arr &= ~(1ull << d);
+        if (d < 4) {
+          act = lvl_set(act, d, next_act);
+          fin = lvl_set(fin, d, hit_fin);
+          cnt = lvl_set(cnt, d, 0);
+        }
+        d++;
+        i = j;
+        if (kind == JK_OBJ) {
+          state = KEY;
+        } else {
+          // first element, index 0
+          hit = 0;
+          hit_fin = 0;
+          uint32_t L = d - 1;
+          uint32_t A = L < 4 ? lvl_get(act, L) : 0;
+          if (A) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+              if (!((A >> s) & 1)) continue;
+              const warpjq_step &st = P.steps[P.spath[s].step_off + L];
+              if (st.kind == WARPJQ_STEP_INDEX && st.index == 0) {
+                hit |= 1u << s;
+                if (P.spath[s].nsteps == L + 1) hit_fin |= 1u << s;
+              }
+            }
+          }
+          state = VALUE;
+        }
+        continue;
+      }
+      uint32_t j = i;
+      bool ok;
+      if (kind == JK_STR) {
+        bool esc = false;
+        ok = scan_string(rd, j, esc);
+      } else if (kind == JK_NUM) {
+        ok = scan_number(rd, j);
+      } else {
+        ok = scan_lit(rd, j, c);
+      }
+      if (!ok) return ST_INVALID;
+      if (hit_fin) {
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+          if ((hit_fin >> s) & 1) sl[s].len = j - i;
+      }
+      i = j;
+      state = AFTER;
+    } else if (state == KEY) {
+      i = skip_ws(rd, i);
+      if (rd.byte(i) != '"') return ST_INVALID;
+      uint32_t L = d - 1;
+      uint32_t A = L < 4 ? lvl_get(act, L) : 0;
+      uint32_t ks = i + 1;
+      bool esc = false;
+      if (!scan_string(rd, i, esc)) return ST_INVALID;
+      uint32_t ke = i - 1;
+      hit = 0;
+      hit_fin = 0;
+      if (A) {
+        uint32_t dlen;
+        uint32_t h = decoded_hash(rd, ks, ke, esc, &dlen);
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+          if (!((A >> s) & 1)) continue;
+          const warpjq_step &st = P.steps[P.spath[s].step_off + L];
+          if (key_step_match(rd, P, st, ks, ke, esc, h, dlen)) {
+            hit |= 1u << s;
+            if (P.spath[s].nsteps == L + 1) hit_fin |= 1u << s;
+          }
+        }
+      }
+      i = skip_ws(rd, i);
+      if (rd.byte(i) != ':') return ST_INVALID;
+      i++;
+      state = VALUE;
+    } else {  // AFTER
+      if (d == 0) break;
+      i = skip_ws(rd, i);
+      uint32_t c = rd.byte(i);
+      bool top_arr = (arr >> (d - 1)) & 1;
+      if (c == ',') {
+        i++;
+        if (top_arr) {
+          hit = 0;
+          hit_fin = 0;
+          uint32_t L = d - 1;
+          uint32_t A = L < 4 ? lvl_get(act, L) : 0;
+          if (A) {
+            uint32_t k = lvl_get(cnt, L);
+            if (k < 0xFFFF) k++;
+            cnt = lvl_set(cnt, L, k);
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+              if (!((A >> s) & 1)) continue;
+              const warpjq_step &st = P.steps[P.spath[s].step_off + L];
+              if (st.kind == WARPJQ_STEP_INDEX && st.index == k) {
+                hit |= 1u << s;
+                if (P.spath[s].nsteps == L + 1) hit_fin |= 1u << s;
+              }
+            }
+          }
+          state = VALUE;
+        } else {
+          state = KEY;
+        }
+      } else if ((c == '}' && !top_arr) || (c == ']' && top_arr)) {
+        i++;
+        d--;
+        if (d < 4) {
+          uint32_t F = lvl_get(fin, d);
+          if (F) {
+#pragma unroll
+            for (int s = 0; s < NS; s++)
+              if ((F >> s) & 1) sl[s].len = i - sl[s].off;
+          }
+          act = lvl_set(act, d, 0);
+          fin = lvl_set(fin, d, 0);
+        }
+        state = AFTER;
+      } else {
+        return ST_INVALID;
+      }
+    }
+  }
+  i = skip_ws(rd, i);
+  if (rd.byte(i) != WJQ_END) return ST_INVALID;
+  end = i;
+  return ST_OK;
+}
+
+// Skips one value of an already-validated line; returns one past it.
+template <class R>
+WJQ_HD uint32_t skip_valid(R &rd, uint32_t i) {
+  uint32_t c = rd.byte(i);
+  if (c == '"') {
+    bool esc;
+    scan_string(rd, i, esc);
+    return i;
+  }
+  if (c == '{' || c == '[') {
+    uint32_t depth = 0;
+    for (;;) {
+      c = rd.byte(i);
+      if (c == '"') {
+        bool esc;
+        scan_string(rd, i, esc);
+        continue;
+      }
+      if (c == '{' || c == '[') depth++;
+      else if (c == '}' || c == ']') {
+        depth--;
+        if (depth == 0) return i + 1;
+      }
+      i++;
+    }
+  }
+  // scalar: ends at a delimiter or whitespace
+  for (;;) {
+    c = rd.byte(i);
+    if (c == ',' || c == '}' || c == ']' || is_jws(c) || c == WJQ_END) return i;
+    i++;
+  }
+}
+
+// Sequential lookup (json.rs:444-508) on a validated line, for paths longer
+// than the walk captures. Returns false for a type error.
+template <class R>
+WJQ_HD bool lookup_valid(R &rd, uint32_t line, const DProg &P, const SlotPath &sp, SlotV &out) {
+  uint32_t start = skip_ws(rd, line);
+  uint32_t kind = kind_of(rd.byte(start));
+  uint32_t endp = 0;
+  for (uint32_t k = 0; k < sp.nsteps; k++) {
+    if (kind == JK_NULL) {
+      out.kind = JK_MISSING;
+      return true;
+    }
+    const warpjq_step &st = P.steps[sp.step_off + k];
+    bool found = false;
+    uint32_t vs = 0, ve = 0;
+    if (st.kind == WARPJQ_STEP_KEY) {
+      if (kind != JK_OBJ) return false;
+      uint32_t j = skip_ws(rd, start + 1);
+      if (rd.byte(j) != '}') {
+        for (;;) {
+          j = skip_ws(rd, j);
+          uint32_t ks = j + 1;
+          bool esc = false;
+          scan_string(rd, j, esc);
+          uint32_t ke = j - 1;
+          j = skip_ws(rd, j) + 1;  // ':'
+          uint32_t s = skip_ws(rd, j);
+          uint32_t e = skip_valid(rd, s);
+          uint32_t dlen;
+          uint32_t h = decoded_hash(rd, ks, ke, esc, &dlen);
+          if (dlen == st.key_len && h == st.key_hash &&
+              decoded_cmp(rd, ks, ke, esc, P.blob + st.key_off, st.key_len) == 0) {
+            found = true;
+            vs = s;
+            ve = e;
+          }
+          j = skip_ws(rd, e);
+          if (rd.byte(j) == ',') {
+            j++;
+            continue;
+          }
+          break;
+        }
+      }
+    } else {
+      if (kind != JK_ARR) return false;
+      uint32_t j = skip_ws(rd, start + 1);
+      if (rd.byte(j) != ']') {
+        uint32_t n = 0;
+        for (;;) {
+          uint32_t s = skip_ws(rd, j);
+          uint32_t e = skip_valid(rd, s);
+          if (n == st.index) {
+            found = true;
+            vs = s;
+            ve = e;
+            break;
+          }
+          n++;
+          j = skip_ws(rd, e);
+          if (rd.byte(j) == ',') {
+            j++;
+            continue;
+          }
+          break;
+        }
+      }
+    }
+    if (!found) {
+      out.kind = JK_MISSING;
+      return true;
+    }
+    start = vs;
+    endp = ve;
+    kind = kind_of(rd.byte(vs));
+  }
+  out.kind = kind;
+  out.off = start;
+  out.len = endp - start;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// predicate (cpu.rs:443-453, json.rs:662-693, ir.rs:66-76)
+// ---------------------------------------------------------------------------
+
+template <class R>
+WJQ_HD uint32_t slot_rank(R &rd, const SlotV &s) {
+  switch (s.kind) {
+    case JK_MISSING:
+    case JK_NULL: return 0;
+    case JK_BOOL: return rd.byte(s.off) == 't' ? 2 : 1;
+    default: return s.kind;  // NUM 3, STR 4, ARR 5, OBJ 6
+  }
+}
+
+// Returns -1/0/1, or 2 when a number could not be decided.
+template <class R>
+WJQ_HD int compare_lit(R &rd, const DProg &P, const SlotV &s, const warpjq_cmp &c) {
+  uint32_t lrank = c.lit_kind;  // NULL 0, FALSE 1, TRUE 2, NUM 3, STR 4
+  uint32_t srank = slot_rank(rd, s);
+  if (srank != lrank) return srank < lrank ? -1 : 1;
+  if (lrank == WARPJQ_LIT_NUM) {
+    NumVal nv;
+    if (!parse_number(rd, s.off, s.off + s.len, nv)) return 2;
+    if (nv.v < c.lit_num) return -1;
+    if (nv.v > c.lit_num) return 1;
+    return 0;
+  }
+  if (lrank == WARPJQ_LIT_STR) {
+    bool esc = body_has_escape(rd, s.off + 1, s.off + s.len - 1);
+    return decoded_cmp(rd, s.off + 1, s.off + s.len - 1, esc, P.blob + c.lit_off, c.lit_len);
+  }
+  return 0;
+}
+
+WJQ_HD bool op_accepts(uint32_t op, int ord) {
+  switch (op) {
+    case WARPJQ_OP_EQ: return ord == 0;
+    case WARPJQ_OP_NE: return ord != 0;
+    case WARPJQ_OP_LT: return ord < 0;
+    case WARPJQ_OP_LE: return ord <= 0;
+    case WARPJQ_OP_GT: return ord > 0;
+    default: return ord >= 0;
+  }
+}
+
+template <class R>
+WJQ_HD bool truthy(R &rd, const SlotV &s) {
+  if (s.kind == JK_MISSING || s.kind == JK_NULL) return false;
+  return !(s.kind == JK_BOOL && rd.byte(s.off) == 'f');
+}
+
+// Evaluates the RPN filter. Returns 0/1, or 2 when undecided.
+template <int NS, class R>
+WJQ_HD int eval_cond(R &rd, const DProg &P, const SlotV (&sl)[NS]) {
+  uint32_t stack = 0, sp = 0;
+  for (uint32_t k = 0; k < P.n_cond; k++) {
+    const warpjq_cond_op o = P.cond[k];
+    uint32_t v;
+    if (o.op == WARPJQ_COND_CMP) {
+      const warpjq_cmp &c = P.cmps[o.arg];
+      uint32_t s = P.slot_of_path[c.path];
+      SlotV x = sl[0];
+#pragma unroll
+      for (int q = 0; q < NS; q++)
+        if ((uint32_t)q == s) x = sl[q];
+      int ord = compare_lit(rd, P, x, c);
+      if (ord == 2) return 2;
+      v = op_accepts(c.op, ord) ? 1u : 0u;
+    } else if (o.op == WARPJQ_COND_TRUTHY) {
+      uint32_t s = P.slot_of_path[o.arg];
+      SlotV x = sl[0];
+#pragma unroll
+      for (int q = 0; q < NS; q++)
+        if ((uint32_t)q == s) x = sl[q];
+      v = truthy(rd, x) ? 1u : 0u;
+    } else if (o.op == WARPJQ_COND_NOT) {
+      stack ^= 1u << (sp - 1);
+      continue;
+    } else {
+      uint32_t b = (stack >> (sp - 1)) & 1, a = (stack >> (sp - 2)) & 1;
+      sp -= 2;
+      v = o.op == WARPJQ_COND_AND ? (a & b) : (a | b);
+    }
+    stack = (stack & ~(1u << sp)) | (v << sp);
+    sp++;
+  }
+  return (int)(stack & 1);
+}
+
+// ---------------------------------------------------------------------------
+// one line, in the CPU engine's order (cpu.rs:308-338)
+// ---------------------------------------------------------------------------
+
+struct LineResult {
+  uint32_t status;
+  uint32_t selected;  // passed the filter and every post path resolved
+  uint32_t end;       // position of the terminator (valid lines)
+};
+
+template <int NS, class R>
+WJQ_HD LineResult eval_line(R &rd, uint32_t s, const DProg &P, SlotV (&sl)[NS]) {
+  LineResult r{ST_OK, 0, 0};
+  uint32_t i = s;
+  bool ff = false;
+  for (;;) {
+    uint32_t c = rd.byte(i);
+    if (is_jws(c)) {
+      i++;
+    } else if (c == 0x0C) {
+      ff = true;
+      i++;
+    } else {
+      break;
+    }
+  }
+  if (rd.byte(i) == WJQ_END) {
+    r.status = ST_BLANK;
+    return r;
+  }
+  if (ff) {
+    r.status = ST_INVALID;
+    return r;
+  }
+#pragma unroll
+  for (int q = 0; q < NS; q++) {
+    sl[q].off = 0;
+    sl[q].len = 0;
+    sl[q].kind = JK_MISSING;
+  }
+  uint32_t te = 0;
+  int st = walk<NS>(rd, i, P, sl, te, r.end);
+  if (st != ST_OK) {
+    r.status = (uint32_t)st;
+    return r;
+  }
+  uint32_t generic = ((1u << P.n_slots) - 1) & ~P.fused_mask;
+  if (generic) {
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+      if (!((generic >> q) & 1)) continue;
+      if (!lookup_valid(rd, i, P, P.spath[q], sl[q])) te |= 1u << q;
+    }
+  }
+  if (te & P.filter_mask) {
+    r.status = ST_TYPE_ERROR;
+    return r;
+  }
+  if (P.has_filter) {
+    int k = eval_cond<NS>(rd, P, sl);
+    if (k == 2) {
+      r.status = ST_FALLBACK;
+      return r;
+    }
+    if (!k) return r;
+  }
+  if (te & P.post_mask) {
+    r.status = ST_TYPE_ERROR;
+    return r;
+  }
+  r.selected = 1;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// rows (output.rs:204-264): the length of the row for a selected line, and
+// its bytes, through a byte sink `W` with put(uint8_t).
+// ---------------------------------------------------------------------------
+
+WJQ_HD bool csv_special(uint32_t c) { return c == ',' || c == '"' || c == '\n' || c == '\r'; }
+
+// CSV cell for one value (output.rs:65-81). Writes when `w` is non-null;
+// returns the length either way.
+template <class R, class W>
+WJQ_HD uint64_t csv_cell(R &rd, const SlotV &s, W *w) {
+  if (s.kind == JK_MISSING || s.kind == JK_NULL) return 0;
+  uint32_t a = s.off, b = s.off + s.len;
+  bool esc = false;
+  if (s.kind == JK_STR) {
+    a++;
+    b--;
+    esc = body_has_escape(rd, a, b);
+  }
+  // pass 1: length and whether quoting is needed
+  uint64_t n = 0, quotes = 0;
+  bool q = false;
+  {
+    DecIter<R> it(rd, a, esc ? b : a);
+    if (esc) {
+      for (int c; (c = it.next()) >= 0; n++) {
+        if (csv_special((uint32_t)c)) q = true;
+        if (c == '"') quotes++;
+      }
+    } else {
+      for (uint32_t p = a; p < b; p++, n++) {
+        uint32_t c = rd.byte(p);
+        if (csv_special(c)) q = true;
+        if (c == '"') quotes++;
+      }
+    }
+  }
+  uint64_t len = q ? n + quotes + 2 : n;
+  if (!w) return len;
+  if (q) w->put('"');
+  if (esc) {
+    DecIter<R> it(rd, a, b);
+    for (int c; (c = it.next()) >= 0;) {
+      if (q && c == '"') w->put('"');
+      w->put((uint8_t)c);
+    }
+  } else {
+    for (uint32_t p = a; p < b; p++) {
+      uint32_t c = rd.byte(p);
+      if (q && c == '"') w->put('"');
+      w->put((uint8_t)c);
+    }
+  }
+  if (q) w->put('"');
+  return len;
+}
+
+template <int NS, class R, class W>
+WJQ_HD uint64_t put_value(R &rd, const DProg &P, const SlotV &s, W *w) {
+  if (P.csv_mode) return csv_cell(rd, s, w);
+  if (s.kind == JK_MISSING) {
+    if (w) {
+      w->put('n');
+      w->put('u');
+      w->put('l');
+      w->put('l');
+    }
+    return 4;
+  }
+  if (w) w->span(rd, s.off, s.len);
+  return s.len;
+}
+
+template <int NS>
+WJQ_HD SlotV pick(const SlotV (&sl)[NS], uint32_t s) {
+  SlotV x = sl[0];
+#pragma unroll
+  for (int q = 0; q < NS; q++)
+    if ((uint32_t)q == s) x = sl[q];
+  return x;
+}
+
+// Row for a selected line starting at `line` with terminator at `end`.
+template <int NS, class R, class W>
+WJQ_HD uint64_t row(R &rd, const DProg &P, const SlotV (&sl)[NS], uint32_t line, uint32_t end, W *w) {
+  uint64_t n = 0;
+  if (P.output_kind == WARPJQ_OUT_PASSTHROUGH) {
+    uint32_t e = end;
+    if (e > line && rd.byte(e - 1) == '\r') e--;
+    if (P.csv_mode) {
+      SlotV whole{line, e - line, JK_NUM};  // raw text as one cell
+      n = csv_cell(rd, whole, w);
+    } else {
+      if (w) w->span(rd, line, e - line);
+      n = e - line;
+    }
+  } else if (P.output_kind == WARPJQ_OUT_PATH) {
+    n = put_value<NS>(rd, P, pick<NS>(sl, P.output_slot), w);
+  } else {
+    for (uint32_t f = 0; f < P.n_fields; f++) {
+      uint32_t pl = P.prefix_len[f];
+      if (w)
+        for (uint32_t k = 0; k < pl; k++) w->put(P.blob[P.prefix_off[f] + k]);
+      n += pl + put_value<NS>(rd, P, pick<NS>(sl, P.field_slot[f]), w);
+    }
+    if (w)
+      for (uint32_t k = 0; k < P.suffix_len; k++) w->put(P.blob[P.suffix_off + k]);
+    n += P.suffix_len;
+  }
+  if (w) w->put('\n');
+  return n + 1;
+}
+
+// Group key of a slot (agg.rs:121-144 and gpu/mod.rs:1198-1221): Missing and
+// null are one group; strings compare by decoded body; everything else by
+// raw bytes. `off/len` are the bytes that identify it.
+struct GKey {
+  uint32_t off, len, kind, esc;
+};
+
+template <class R>
+WJQ_HD GKey group_key(R &rd, const SlotV &s) {
+  GKey k;
+  if (s.kind == JK_MISSING || s.kind == JK_NULL) {
+    k.off = 0;
+    k.len = 0;
+    k.kind = JK_NULL;
+    k.esc = 0;
+  } else if (s.kind == JK_STR) {
+    k.off = s.off + 1;
+    k.len = s.len - 2;
+    k.kind = JK_STR;
+    k.esc = body_has_escape(rd, k.off, k.off + k.len) ? 1 : 0;
+  } else {
+    // bools compare by length (true 4, false 5); numbers and composites by
+    // raw bytes, arrays and objects forming one composite class
+    k.off = s.off;
+    k.len = s.len;
+    k.kind = s.kind == JK_OBJ ? (uint32_t)JK_ARR : s.kind;
+    k.esc = 0;
+  }
+  return k;
+}
+
+template <class R>
+WJQ_HD uint32_t group_hash(R &rd, const GKey &k) {
+  uint32_t h;
+  if (k.kind == JK_STR) h = decoded_hash(rd, k.off, k.off + k.len, k.esc != 0, nullptr);
+  else if (k.kind == JK_NULL || k.kind == JK_BOOL) h = 0x811c9dc5u ^ k.len;
+  else h = decoded_hash(rd, k.off, k.off + k.len, false, nullptr);
+  h ^= k.kind * 0x9E3779B9u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  return h | 1u;  // 0 is reserved for "not yet written"
+}
+
+// Equality of two group keys from the same chunk.
+template <class R>
+WJQ_HD bool group_eq(R &rd, const GKey &a, const GKey &b) {
+  if (a.kind != b.kind) return false;
+  if (a.kind == JK_NULL) return true;
+  if (a.kind == JK_BOOL) return a.len == b.len;
+  if (a.kind == JK_STR && (a.esc || b.esc)) {
+    DecIter<R> x(rd, a.off, a.off + a.len), y(rd, b.off, b.off + b.len);
+    for (;;) {
+      int c = x.next(), e = y.next();
+      if (c != e) return false;
+      if (c < 0) return true;
+    }
+  }
+  if (a.len != b.len) return false;
+  for (uint32_t q = 0; q < a.len; q++)
+    if (rd.byte(a.off + q) != rd.byte(b.off + q)) return false;
+  return true;
+}
+
+}  // namespace wjqd

@@ -1,0 +1,4 @@
+// k_main / k_emit instantiated for a 16-entry slot table.
+#include "kernels.h"
+
+WJQ_DEFINE_LAUNCHERS(16)

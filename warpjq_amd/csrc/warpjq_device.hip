@@ -1,0 +1,806 @@
+// warpjq's device ABI (include/warpjq_kernels.h) implemented for MI355X
+// (gfx950): the non-template kernels, and the host side of the ABI. The
+// per-line kernels live in kernels.h and are instantiated per slot-table
+// width in kern_<N>.hip.
+#include "kernels.h"
+
+// ---------------------------------------------------------------------------
+// k_reset
+// ---------------------------------------------------------------------------
+
+__global__ void k_reset(DevCtr *ctr, unsigned long long *tile_state, uint32_t ntiles) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    ctr->n_lines = ctr->n_blank = ctr->n_invalid = ctr->n_type_error = ctr->n_fallback = 0;
+    ctr->n_sel = ctr->out_total = 0;
+    ctr->first_invalid = 0xFFFFFFFFu;
+    ctr->overflow = ctr->saw_non_numeric = ctr->group_overflow = 0;
+    ctr->fb_count = ctr->n_groups = ctr->ticket = ctr->tiles_done = 0;
+  }
+  for (uint32_t t = i; t < ntiles; t += gridDim.x * blockDim.x) tile_state[t] = 0;
+}
+
+// Clears the whole group table once, at context creation.
+__global__ void k_group_init(GroupTab g) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.cap; i += gridDim.x * blockDim.x) {
+    g.keys[i] = EMPTY_KEY;
+    g.hash[i] = 0;
+    g.count[i] = g.numeric[i] = g.isum_lo[i] = g.isum_hi[i] = 0;
+    g.fsum[i] = g.fcomp[i] = 0.0;
+    g.mnb[i] = ~0ull;
+    g.mxb[i] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// aggregates: finalisation and compaction
+// ---------------------------------------------------------------------------
+
+struct AggOut {
+  warpjq_agg_partial agg;
+  warpjq_amd_agg_ext ext;
+};
+
+__global__ void k_agg_final(const AggPart *parts, uint32_t nparts, const DevCtr *ctr, AggOut *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long cnt = 0, num = 0, lo = 0, hi = 0;
+  double s = 0.0, c = 0.0, mn = INFINITY, mx = -INFINITY;
+  for (uint32_t i = 0; i < nparts; i++) {
+    const AggPart &p = parts[i];
+    cnt += p.count;
+    num += p.numeric;
+    unsigned long long nlo = lo + p.isum_lo;
+    hi += p.isum_hi + (nlo < lo ? 1 : 0);
+    lo = nlo;
+    double t = s + p.fsum;
+    double bp = t - s;
+    c += (s - (t - bp)) + (p.fsum - bp) + p.fcomp;
+    s = t;
+    if (p.mn < mn) mn = p.mn;
+    if (p.mx > mx) mx = p.mx;
+  }
+  out->ext.isum_lo = lo;
+  out->ext.isum_hi = hi;
+  out->ext.fsum = s;
+  out->ext.fcomp = c;
+  out->agg.count = cnt;
+  out->agg.numeric = num;
+  out->agg.sum = 0.0;  // filled on the host from the exact parts
+  out->agg.min = mn;
+  out->agg.max = mx;
+  out->agg.saw_non_numeric = ctr->saw_non_numeric;
+  out->agg._pad = 0;
+}
+
+__global__ void k_group_compact(GroupTab g, DevCtr *ctr, warpjq_group *out, warpjq_amd_agg_ext *ext) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.cap; i += gridDim.x * blockDim.x) {
+    unsigned long long k = g.keys[i];
+    if (k == EMPTY_KEY) continue;
+    GKey gk = unpack_key(k);
+    uint32_t o = atomicAdd(&ctr->n_groups, 1u);
+    warpjq_group r;
+    r.key_off = gk.off;
+    r.key_len = gk.len;
+    r.key_kind = gk.kind;
+    r._pad = 0;
+    r.count = g.count[i];
+    r.numeric = g.numeric[i];
+    r.sum = 0.0;
+    r.min = g.numeric[i] ? from_ordered(g.mnb[i]) : INFINITY;
+    r.max = g.numeric[i] ? from_ordered(g.mxb[i]) : -INFINITY;
+    out[o] = r;
+    warpjq_amd_agg_ext e;
+    e.isum_lo = g.isum_lo[i];
+    e.isum_hi = g.isum_hi[i];
+    e.fsum = g.fsum[i];
+    e.fcomp = g.fcomp[i];
+    ext[o] = e;
+    g.keys[i] = EMPTY_KEY;
+    g.hash[i] = 0;
+    g.count[i] = g.numeric[i] = g.isum_lo[i] = g.isum_hi[i] = 0;
+    g.fsum[i] = g.fcomp[i] = 0.0;
+    g.mnb[i] = ~0ull;
+    g.mxb[i] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// row outputs: reduce-then-scan over per-line row lengths
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t nlines_capped(const DevCtr *ctr, uint64_t max_lines) {
+  uint64_t n = ctr->n_lines;
+  return n < max_lines ? n : max_lines;
+}
+
+// Block b sums lines [b*1024, (b+1)*1024): selected count and bytes.
+__global__ __launch_bounds__(256) void k_row_reduce(const uint32_t *row_len, const DevCtr *ctr, uint64_t max_lines,
+                                                     uint64_t *blk_cnt, uint64_t *blk_bytes) {
+  uint64_t n = nlines_capped(ctr, max_lines);
+  uint64_t base = (uint64_t)blockIdx.x * ROWSCAN_BLOCK;
+  if (base >= n) return;
+  uint64_t c = 0, b = 0;
+  for (uint32_t k = threadIdx.x; k < ROWSCAN_BLOCK; k += 256) {
+    uint64_t i = base + k;
+    if (i < n) {
+      uint32_t r = row_len[i];
+      c += r ? 1 : 0;
+      b += r;
+    }
+  }
+  __shared__ uint64_t sc[4], sbt[4];
+  c = wave_sum(c);
+  b = wave_sum(b);
+  if ((threadIdx.x & 63) == 0) {
+    sc[threadIdx.x >> 6] = c;
+    sbt[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    blk_cnt[blockIdx.x] = sc[0] + sc[1] + sc[2] + sc[3];
+    blk_bytes[blockIdx.x] = sbt[0] + sbt[1] + sbt[2] + sbt[3];
+  }
+}
+
+// One block: exclusive scan of the block sums; totals into the counters.
+__global__ __launch_bounds__(1024) void k_row_blockscan(uint64_t *blk_cnt, uint64_t *blk_bytes, DevCtr *ctr,
+                                                        uint64_t max_lines, uint64_t out_cap) {
+  uint64_t n = nlines_capped(ctr, max_lines);
+  uint32_t nb = (uint32_t)((n + ROWSCAN_BLOCK - 1) / ROWSCAN_BLOCK);
+  __shared__ uint64_t wc[16], wb[16];
+  __shared__ uint64_t run_c, run_b;
+  if (threadIdx.x == 0) {
+    run_c = 0;
+    run_b = 0;
+  }
+  __syncthreads();
+  for (uint32_t base = 0; base < nb; base += 1024) {
+    uint32_t i = base + threadIdx.x;
+    uint64_t c = i < nb ? blk_cnt[i] : 0, b = i < nb ? blk_bytes[i] : 0;
+    // inclusive wave scan
+    uint64_t xc = c, xb = b;
+    uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      uint64_t yc = __shfl_up(xc, o, 64), yb = __shfl_up(xb, o, 64);
+      if (lane >= (uint32_t)o) {
+        xc += yc;
+        xb += yb;
+      }
+    }
+    if (lane == 63) {
+      wc[threadIdx.x >> 6] = xc;
+      wb[threadIdx.x >> 6] = xb;
+    }
+    __syncthreads();
+    uint64_t pc = run_c, pb = run_b;
+    for (uint32_t w = 0; w < (threadIdx.x >> 6); w++) {
+      pc += wc[w];
+      pb += wb[w];
+    }
+    if (i < nb) {
+      blk_cnt[i] = pc + xc - c;
+      blk_bytes[i] = pb + xb - b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (uint32_t w = 0; w < 16; w++) {
+        run_c += wc[w];
+        run_b += wb[w];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+      wc[threadIdx.x] = 0;
+      wb[threadIdx.x] = 0;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    ctr->n_sel = run_c;
+    ctr->out_total = run_b;
+    if (run_b > out_cap || ctr->n_lines > max_lines) ctr->overflow = 1;
+  }
+}
+
+// Block b writes the selected indices and row offsets of its lines.
+__global__ __launch_bounds__(256) void k_row_scatter(const uint32_t *row_len, const DevCtr *ctr, uint64_t max_lines,
+                                                      const uint64_t *blk_cnt, const uint64_t *blk_bytes,
+                                                      uint32_t *sel_idx, uint64_t *row_off) {
+  uint64_t n = nlines_capped(ctr, max_lines);
+  uint64_t base = (uint64_t)blockIdx.x * ROWSCAN_BLOCK;
+  if (base >= n) return;
+  __shared__ uint64_t wc[4], wb[4];
+  __shared__ uint64_t run_c, run_b;
+  if (threadIdx.x == 0) {
+    run_c = blk_cnt[blockIdx.x];
+    run_b = blk_bytes[blockIdx.x];
+  }
+  __syncthreads();
+  uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (uint32_t k0 = 0; k0 < ROWSCAN_BLOCK; k0 += 256) {
+    uint64_t i = base + k0 + threadIdx.x;
+    uint32_t r = i < n ? row_len[i] : 0;
+    uint64_t c = r ? 1 : 0, b = r;
+    uint64_t xc = c, xb = b;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      uint64_t yc = __shfl_up(xc, o, 64), yb = __shfl_up(xb, o, 64);
+      if (lane >= (uint32_t)o) {
+        xc += yc;
+        xb += yb;
+      }
+    }
+    if (lane == 63) {
+      wc[w] = xc;
+      wb[w] = xb;
+    }
+    __syncthreads();
+    uint64_t pc = run_c, pb = run_b;
+    for (uint32_t q = 0; q < w; q++) {
+      pc += wc[q];
+      pb += wb[q];
+    }
+    if (r) {
+      uint64_t sidx = pc + xc - c;
+      sel_idx[sidx] = (uint32_t)i;
+      row_off[sidx] = pb + xb - b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      run_c += wc[0] + wc[1] + wc[2] + wc[3];
+      run_b += wb[0] + wb[1] + wb[2] + wb[3];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_row_finish(const DevCtr *ctr, uint64_t *row_off) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) row_off[ctr->n_sel] = ctr->out_total;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;  // before H2D, after H2D, after kernels
+  uint8_t *h_pinned = nullptr;
+  uint8_t *d_data = nullptr;
+  const uint8_t *d_src = nullptr;  // what the last submit read (staging or a resident buffer)
+  uint64_t n_bytes = 0;
+  bool resident = false;
+
+  DevCtr *d_ctr = nullptr;
+  DevCtr *h_ctr = nullptr;
+  unsigned long long *d_tile_state = nullptr;
+  AggPart *d_parts = nullptr;
+  AggOut *d_aggout = nullptr, *h_aggout = nullptr;
+
+  GroupTab gt{};
+  warpjq_group *d_groups = nullptr, *h_groups = nullptr;
+  warpjq_amd_agg_ext *d_gext = nullptr, *h_gext = nullptr;
+
+  uint32_t *d_row_len = nullptr, *d_sel_start = nullptr, *d_sel_idx = nullptr;
+  uint64_t *d_row_off = nullptr, *d_blk_cnt = nullptr, *d_blk_bytes = nullptr;
+  uint8_t *d_out = nullptr;
+  uint32_t *h_sel_idx = nullptr;
+  uint64_t *h_row_off = nullptr;
+  uint8_t *h_out = nullptr;
+
+  uint32_t *d_fb_idx = nullptr, *d_fb_off = nullptr, *d_fb_len = nullptr;
+  uint32_t *h_fb_idx = nullptr, *h_fb_off = nullptr, *h_fb_len = nullptr;
+
+  bool submitted = false;
+};
+
+}  // namespace
+
+struct warpjq_ctx {
+  DProgHost host;
+  // device copies of the program tables
+  void *d_tables = nullptr;
+  DProg dprog{};
+  Slot *slots = nullptr;
+  uint32_t n_slots = 0;
+  uint64_t chunk_cap = 0;
+  uint64_t max_lines = 0;
+  uint32_t fb_cap = 0;
+  uint64_t out_cap = 0;
+  uint32_t group_cap = 0;
+  uint32_t grid = 0;
+  uint32_t max_tiles = 0;
+  bool grouped = false, rows = false;
+  int device = 0;
+};
+
+static void set_err(char *err, size_t len, const char *op, const char *detail) {
+  if (err && len) snprintf(err, len, "%s: %s", op, detail);
+}
+
+#define HIP_TRY(expr, op)                                      \
+  do {                                                         \
+    hipError_t _e = (expr);                                    \
+    if (_e != hipSuccess) {                                    \
+      set_err(err, err_len, op, hipGetErrorString(_e));        \
+      return _e == hipErrorOutOfMemory ? WARPJQ_ERR_OOM : WARPJQ_ERR_CUDA; \
+    }                                                          \
+  } while (0)
+
+template <class T>
+static hipError_t dalloc(T **p, size_t n) {
+  if (n == 0) n = 1;
+  return hipMalloc((void **)p, n * sizeof(T));
+}
+template <class T>
+static hipError_t halloc(T **p, size_t n) {
+  if (n == 0) n = 1;
+  return hipHostMalloc((void **)p, n * sizeof(T), hipHostMallocDefault);
+}
+
+static void free_slot(Slot &s) {
+  if (s.stream) (void)hipStreamSynchronize(s.stream);
+  (void)hipFree(s.d_data);
+  (void)hipHostFree(s.h_pinned);
+  (void)hipFree(s.d_ctr);
+  (void)hipHostFree(s.h_ctr);
+  (void)hipFree(s.d_tile_state);
+  (void)hipFree(s.d_parts);
+  (void)hipFree(s.d_aggout);
+  (void)hipHostFree(s.h_aggout);
+  (void)hipFree(s.gt.keys);
+  (void)hipFree(s.gt.hash);
+  (void)hipFree(s.gt.count);
+  (void)hipFree(s.gt.numeric);
+  (void)hipFree(s.gt.isum_lo);
+  (void)hipFree(s.gt.isum_hi);
+  (void)hipFree(s.gt.mnb);
+  (void)hipFree(s.gt.mxb);
+  (void)hipFree(s.gt.fsum);
+  (void)hipFree(s.gt.fcomp);
+  (void)hipFree(s.d_groups);
+  (void)hipHostFree(s.h_groups);
+  (void)hipFree(s.d_gext);
+  (void)hipHostFree(s.h_gext);
+  (void)hipFree(s.d_row_len);
+  (void)hipFree(s.d_sel_start);
+  (void)hipFree(s.d_sel_idx);
+  (void)hipFree(s.d_row_off);
+  (void)hipFree(s.d_blk_cnt);
+  (void)hipFree(s.d_blk_bytes);
+  (void)hipFree(s.d_out);
+  (void)hipHostFree(s.h_sel_idx);
+  (void)hipHostFree(s.h_row_off);
+  (void)hipHostFree(s.h_out);
+  (void)hipFree(s.d_fb_idx);
+  (void)hipFree(s.d_fb_off);
+  (void)hipFree(s.d_fb_len);
+  (void)hipHostFree(s.h_fb_idx);
+  (void)hipHostFree(s.h_fb_off);
+  (void)hipHostFree(s.h_fb_len);
+  if (s.ev0) (void)hipEventDestroy(s.ev0);
+  if (s.ev1) (void)hipEventDestroy(s.ev1);
+  if (s.ev2) (void)hipEventDestroy(s.ev2);
+  if (s.stream) (void)hipStreamDestroy(s.stream);
+  s = Slot{};
+}
+
+static warpjq_status alloc_slot(warpjq_ctx *ctx, Slot &s, char *err, size_t err_len) {
+  HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "hipStreamCreate");
+  HIP_TRY(hipEventCreate(&s.ev0), "hipEventCreate");
+  HIP_TRY(hipEventCreate(&s.ev1), "hipEventCreate");
+  HIP_TRY(hipEventCreate(&s.ev2), "hipEventCreate");
+  HIP_TRY(halloc(&s.h_pinned, ctx->chunk_cap), "hipHostMalloc(staging)");
+  HIP_TRY(dalloc(&s.d_data, ctx->chunk_cap + DATA_PAD), "hipMalloc(data)");
+  HIP_TRY(hipMemsetAsync(s.d_data, 0, ctx->chunk_cap + DATA_PAD, s.stream), "hipMemset(data)");
+  HIP_TRY(dalloc(&s.d_ctr, 1), "hipMalloc(ctr)");
+  HIP_TRY(halloc(&s.h_ctr, 1), "hipHostMalloc(ctr)");
+  HIP_TRY(dalloc(&s.d_tile_state, ctx->max_tiles), "hipMalloc(tile_state)");
+  HIP_TRY(dalloc(&s.d_parts, ctx->grid), "hipMalloc(parts)");
+  HIP_TRY(dalloc(&s.d_aggout, 1), "hipMalloc(agg)");
+  HIP_TRY(halloc(&s.h_aggout, 1), "hipHostMalloc(agg)");
+  const uint32_t fc = ctx->fb_cap;
+  HIP_TRY(dalloc(&s.d_fb_idx, fc), "hipMalloc(fb)");
+  HIP_TRY(dalloc(&s.d_fb_off, fc), "hipMalloc(fb)");
+  HIP_TRY(dalloc(&s.d_fb_len, fc), "hipMalloc(fb)");
+  HIP_TRY(halloc(&s.h_fb_idx, fc), "hipHostMalloc(fb)");
+  HIP_TRY(halloc(&s.h_fb_off, fc), "hipHostMalloc(fb)");
+  HIP_TRY(halloc(&s.h_fb_len, fc), "hipHostMalloc(fb)");
+  if (ctx->grouped) {
+    const uint32_t G = ctx->group_cap;
+    s.gt.cap = G;
+    HIP_TRY(dalloc(&s.gt.keys, G), "hipMalloc(group keys)");
+    HIP_TRY(dalloc(&s.gt.hash, G), "hipMalloc(group hash)");
+    HIP_TRY(dalloc(&s.gt.count, G), "hipMalloc(group)");
+    HIP_TRY(dalloc(&s.gt.numeric, G), "hipMalloc(group)");
+    HIP_TRY(dalloc(&s.gt.isum_lo, G), "hipMalloc(group)");
+    HIP_TRY(dalloc(&s.gt.isum_hi, G), "hipMalloc(group)");
+    HIP_TRY(dalloc(&s.gt.mnb, G), "hipMalloc(group)");
+    HIP_TRY(dalloc(&s.gt.mxb, G), "hipMalloc(group)");
+    HIP_TRY(dalloc(&s.gt.fsum, G), "hipMalloc(group)");
+    HIP_TRY(dalloc(&s.gt.fcomp, G), "hipMalloc(group)");
+    HIP_TRY(dalloc(&s.d_groups, G), "hipMalloc(groups)");
+    HIP_TRY(halloc(&s.h_groups, G), "hipHostMalloc(groups)");
+    HIP_TRY(dalloc(&s.d_gext, G), "hipMalloc(groups)");
+    HIP_TRY(halloc(&s.h_gext, G), "hipHostMalloc(groups)");
+    hipLaunchKernelGGL(k_group_init, dim3(1024), dim3(256), 0, s.stream, s.gt);
+    HIP_TRY(hipGetLastError(), "k_group_init");
+  }
+  if (ctx->rows) {
+    const uint64_t ml = ctx->max_lines;
+    const uint64_t nb = (ml + ROWSCAN_BLOCK - 1) / ROWSCAN_BLOCK;
+    HIP_TRY(dalloc(&s.d_row_len, ml), "hipMalloc(row_len)");
+    HIP_TRY(dalloc(&s.d_sel_start, ml), "hipMalloc(sel_start)");
+    HIP_TRY(dalloc(&s.d_sel_idx, ml), "hipMalloc(sel_idx)");
+    HIP_TRY(dalloc(&s.d_row_off, ml + 1), "hipMalloc(row_off)");
+    HIP_TRY(dalloc(&s.d_blk_cnt, nb), "hipMalloc(blk)");
+    HIP_TRY(dalloc(&s.d_blk_bytes, nb), "hipMalloc(blk)");
+    HIP_TRY(dalloc(&s.d_out, ctx->out_cap), "hipMalloc(out)");
+    HIP_TRY(halloc(&s.h_sel_idx, ml), "hipHostMalloc(sel_idx)");
+    HIP_TRY(halloc(&s.h_row_off, ml + 1), "hipHostMalloc(row_off)");
+    HIP_TRY(halloc(&s.h_out, ctx->out_cap), "hipHostMalloc(out)");
+  }
+  HIP_TRY(hipStreamSynchronize(s.stream), "init sync");
+  return WARPJQ_OK;
+}
+
+template <class T>
+static size_t put_table(std::vector<uint8_t> &buf, const T *src, size_t n) {
+  size_t off = (buf.size() + 15) & ~(size_t)15;
+  buf.resize(off + n * sizeof(T) + 16);
+  if (n) memcpy(buf.data() + off, src, n * sizeof(T));
+  return off;
+}
+
+static uint32_t g_num_cus = 0;
+
+extern "C" {
+
+warpjq_status warpjq_probe(char *err, size_t err_len) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) {
+    set_err(err, err_len, "no usable HIP device", e != hipSuccess ? hipGetErrorString(e) : "device count is zero");
+    return WARPJQ_ERR_NO_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, 0);
+  if (e != hipSuccess) {
+    set_err(err, err_len, "hipGetDeviceProperties", hipGetErrorString(e));
+    return WARPJQ_ERR_NO_DEVICE;
+  }
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_err(err, err_len, "unsupported GPU", "this build targets gfx950 (MI355X) only");
+    return WARPJQ_ERR_NO_DEVICE;
+  }
+  return WARPJQ_OK;
+}
+
+warpjq_status warpjq_device_name(char *buf, size_t len) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return WARPJQ_ERR_NO_DEVICE;
+  snprintf(buf, len, "%s (%s, %d CUs, %zu MiB)", prop.name, prop.gcnArchName, prop.multiProcessorCount,
+           (size_t)(prop.totalGlobalMem >> 20));
+  return WARPJQ_OK;
+}
+
+warpjq_status warpjq_abi_check(const uint64_t *sizes, size_t n) {
+  const uint64_t mine[] = {sizeof(warpjq_step),        sizeof(warpjq_path),  sizeof(warpjq_cmp),
+                           sizeof(warpjq_cond_op),     sizeof(warpjq_agg_partial), sizeof(warpjq_group)};
+  if (!sizes || n != sizeof(mine) / sizeof(mine[0])) return WARPJQ_ERR_ABI;
+  for (size_t i = 0; i < n; i++)
+    if (sizes[i] != mine[i]) return WARPJQ_ERR_ABI;
+  return WARPJQ_OK;
+}
+
+warpjq_status warpjq_ctx_create(const warpjq_program *prog, uint64_t max_chunk_bytes, uint32_t n_slots,
+                                warpjq_ctx **out, char *err, size_t err_len) {
+  if (!prog || !out || n_slots == 0 || max_chunk_bytes == 0) return WARPJQ_ERR_INVALID_ARG;
+  if (max_chunk_bytes > MAX_CHUNK) {
+    set_err(err, err_len, "chunk is too large", "chunk offsets are 32-bit");
+    return WARPJQ_ERR_INVALID_ARG;
+  }
+  warpjq_ctx *ctx = new (std::nothrow) warpjq_ctx();
+  if (!ctx) return WARPJQ_ERR_OOM;
+  std::string why;
+  if (!build_dprog(*prog, ctx->host, why)) {
+    set_err(err, err_len, "query is not supported by the kernels", why.c_str());
+    delete ctx;
+    return WARPJQ_ERR_INVALID_ARG;
+  }
+  hipGetDevice(&ctx->device);
+  if (!g_num_cus) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) g_num_cus = (uint32_t)prop.multiProcessorCount;
+    if (!g_num_cus) g_num_cus = 256;
+  }
+  ctx->n_slots = n_slots;
+  ctx->chunk_cap = max_chunk_bytes;
+  ctx->max_lines = max_chunk_bytes / MIN_LINE_BYTES + 2;
+  ctx->fb_cap = (uint32_t)(ctx->max_lines / 16 + 1024);
+  ctx->out_cap = max_chunk_bytes + max_chunk_bytes / 2 + (1u << 20);
+  ctx->grouped = prog->output_kind == WARPJQ_OUT_AGG && prog->group_path != 0xFFFFFFFFu;
+  ctx->rows = prog->output_kind != WARPJQ_OUT_AGG;
+  uint64_t g = 65536;
+  while (g < max_chunk_bytes / 128 && g < (1u << 22)) g <<= 1;
+  ctx->group_cap = (uint32_t)g;
+  ctx->grid = g_num_cus * 16;
+  // tiles for the largest chunk a slot or a resident buffer may hold
+  ctx->max_tiles = (uint32_t)((MAX_CHUNK + TILE_BYTES - 1) / TILE_BYTES) + 1;
+
+  // one device allocation holding every program table
+  std::vector<uint8_t> buf;
+  const DProg &h = ctx->host.d;
+  size_t o_steps = put_table(buf, prog->steps, prog->n_steps);
+  size_t o_cmps = put_table(buf, prog->cmps, prog->n_cmps);
+  size_t o_cond = put_table(buf, prog->cond_rpn, prog->n_cond);
+  size_t o_blob = put_table(buf, prog->blob, prog->n_blob);
+  size_t o_spath = put_table(buf, ctx->host.spath.data(), ctx->host.spath.size());
+  size_t o_sop = put_table(buf, prog->slot_of_path, prog->n_paths);
+  size_t o_fs = put_table(buf, ctx->host.field_slot.data(), ctx->host.field_slot.size());
+  size_t o_po = put_table(buf, prog->prefix_off, prog->n_fields);
+  size_t o_pl = put_table(buf, prog->prefix_len, prog->n_fields);
+  hipError_t e = hipMalloc(&ctx->d_tables, buf.size());
+  if (e == hipSuccess) e = hipMemcpy(ctx->d_tables, buf.data(), buf.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_err(err, err_len, "uploading the compiled query", hipGetErrorString(e));
+    warpjq_ctx_destroy(ctx);
+    return WARPJQ_ERR_CUDA;
+  }
+  uint8_t *base = (uint8_t *)ctx->d_tables;
+  DProg d = h;
+  d.steps = (const warpjq_step *)(base + o_steps);
+  d.cmps = (const warpjq_cmp *)(base + o_cmps);
+  d.cond = (const warpjq_cond_op *)(base + o_cond);
+  d.blob = base + o_blob;
+  d.spath = (const SlotPath *)(base + o_spath);
+  d.slot_of_path = (const uint32_t *)(base + o_sop);
+  d.field_slot = (const uint32_t *)(base + o_fs);
+  d.prefix_off = (const uint32_t *)(base + o_po);
+  d.prefix_len = (const uint32_t *)(base + o_pl);
+  ctx->dprog = d;
+
+  ctx->slots = new (std::nothrow) Slot[n_slots];
+  if (!ctx->slots) {
+    warpjq_ctx_destroy(ctx);
+    return WARPJQ_ERR_OOM;
+  }
+  for (uint32_t s = 0; s < n_slots; s++) {
+    warpjq_status st = alloc_slot(ctx, ctx->slots[s], err, err_len);
+    if (st != WARPJQ_OK) {
+      warpjq_ctx_destroy(ctx);
+      return st;
+    }
+  }
+  *out = ctx;
+  return WARPJQ_OK;
+}
+
+void warpjq_ctx_destroy(warpjq_ctx *ctx) {
+  if (!ctx) return;
+  if (ctx->slots) {
+    for (uint32_t i = 0; i < ctx->n_slots; i++) free_slot(ctx->slots[i]);
+    delete[] ctx->slots;
+  }
+  (void)hipFree(ctx->d_tables);
+  delete ctx;
+}
+
+uint8_t *warpjq_slot_buffer(warpjq_ctx *ctx, uint32_t slot) {
+  if (!ctx || slot >= ctx->n_slots) return nullptr;
+  return ctx->slots[slot].h_pinned;
+}
+
+uint64_t warpjq_slot_capacity(const warpjq_ctx *ctx) { return ctx ? ctx->chunk_cap : 0; }
+
+uint64_t warpjq_max_lines(const warpjq_ctx *ctx) { return ctx ? ctx->max_lines : 0; }
+
+}  // extern "C"
+
+// Queues the kernel chain for `n` bytes at `src` (device memory) on `s`.
+static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const uint8_t *src, uint64_t n, char *err,
+                                  size_t err_len) {
+  hipStream_t st = s.stream;
+  const uint32_t ntiles = (uint32_t)((n + TILE_BYTES - 1) / TILE_BYTES);
+  hipLaunchKernelGGL(k_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (ntiles + 255) / 256))), dim3(256),
+                     0, st, s.d_ctr, s.d_tile_state, ntiles);
+  if (n > 0) {
+    KArgs a{};
+    a.d = src;
+    a.n = (uint32_t)n;
+    a.ntiles = ntiles;
+    a.P = ctx->dprog;
+    a.ctr = s.d_ctr;
+    a.tile_state = s.d_tile_state;
+    a.parts = s.d_parts;
+    a.gt = s.gt;
+    a.row_len = s.d_row_len;
+    a.sel_start = s.d_sel_start;
+    a.max_lines = ctx->rows ? ctx->max_lines : 0;
+    a.fb_idx = s.d_fb_idx;
+    a.fb_off = s.d_fb_off;
+    a.fb_len = s.d_fb_len;
+    a.fb_cap = ctx->fb_cap;
+    uint32_t grid = std::min<uint32_t>(ctx->grid, ntiles);
+    switch (ctx->host.ns) {
+      case 1: launch_main_1(ctx->grouped, grid, st, a); break;
+      case 2: launch_main_2(ctx->grouped, grid, st, a); break;
+      case 4: launch_main_4(ctx->grouped, grid, st, a); break;
+      case 8: launch_main_8(ctx->grouped, grid, st, a); break;
+      default: launch_main_16(ctx->grouped, grid, st, a); break;
+    }
+    HIP_TRY(hipGetLastError(), "k_main");
+    if (ctx->dprog.output_kind == WARPJQ_OUT_AGG) {
+      if (ctx->grouped) {
+        hipLaunchKernelGGL(k_group_compact, dim3(1024), dim3(256), 0, st, s.gt, s.d_ctr, s.d_groups, s.d_gext);
+      } else {
+        hipLaunchKernelGGL(k_agg_final, dim3(1), dim3(64), 0, st, s.d_parts, grid, s.d_ctr, s.d_aggout);
+      }
+    } else {
+      uint64_t lines_bound = std::min<uint64_t>(ctx->max_lines, n + 1);
+      uint32_t nb = (uint32_t)((lines_bound + ROWSCAN_BLOCK - 1) / ROWSCAN_BLOCK);
+      hipLaunchKernelGGL(k_row_reduce, dim3(nb), dim3(256), 0, st, s.d_row_len, s.d_ctr, ctx->max_lines, s.d_blk_cnt,
+                         s.d_blk_bytes);
+      hipLaunchKernelGGL(k_row_blockscan, dim3(1), dim3(1024), 0, st, s.d_blk_cnt, s.d_blk_bytes, s.d_ctr,
+                         ctx->max_lines, ctx->out_cap);
+      hipLaunchKernelGGL(k_row_scatter, dim3(nb), dim3(256), 0, st, s.d_row_len, s.d_ctr, ctx->max_lines,
+                         s.d_blk_cnt, s.d_blk_bytes, s.d_sel_idx, s.d_row_off);
+      hipLaunchKernelGGL(k_row_finish, dim3(1), dim3(64), 0, st, s.d_ctr, s.d_row_off);
+      uint32_t eg = std::min<uint32_t>(ctx->grid, (uint32_t)((lines_bound + 255) / 256));
+      if (eg == 0) eg = 1;
+      switch (ctx->host.ns) {
+        case 1: launch_emit_1(eg, st, src, (uint32_t)n, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        case 2: launch_emit_2(eg, st, src, (uint32_t)n, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        case 4: launch_emit_4(eg, st, src, (uint32_t)n, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        case 8: launch_emit_8(eg, st, src, (uint32_t)n, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        default: launch_emit_16(eg, st, src, (uint32_t)n, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+      }
+    }
+    HIP_TRY(hipGetLastError(), "kernel chain");
+  }
+  HIP_TRY(hipMemcpyAsync(s.h_ctr, s.d_ctr, sizeof(DevCtr), hipMemcpyDeviceToHost, st), "D2H(counters)");
+  if (ctx->dprog.output_kind == WARPJQ_OUT_AGG && !ctx->grouped)
+    HIP_TRY(hipMemcpyAsync(s.h_aggout, s.d_aggout, sizeof(AggOut), hipMemcpyDeviceToHost, st), "D2H(agg)");
+  HIP_TRY(hipEventRecord(s.ev2, st), "hipEventRecord");
+  s.submitted = true;
+  return WARPJQ_OK;
+}
+
+extern "C" {
+
+warpjq_status warpjq_submit(warpjq_ctx *ctx, uint32_t slot, uint64_t n_bytes, char *err, size_t err_len) {
+  if (!ctx || slot >= ctx->n_slots) return WARPJQ_ERR_INVALID_ARG;
+  if (n_bytes > ctx->chunk_cap) return WARPJQ_ERR_INVALID_ARG;
+  Slot &s = ctx->slots[slot];
+  s.n_bytes = n_bytes;
+  s.resident = false;
+  s.d_src = s.d_data;
+  HIP_TRY(hipEventRecord(s.ev0, s.stream), "hipEventRecord");
+  if (n_bytes) HIP_TRY(hipMemcpyAsync(s.d_data, s.h_pinned, n_bytes, hipMemcpyHostToDevice, s.stream), "H2D chunk");
+  HIP_TRY(hipEventRecord(s.ev1, s.stream), "hipEventRecord");
+  return launch_chain(ctx, s, s.d_data, n_bytes, err, err_len);
+}
+
+warpjq_status warpjq_amd_submit_device(warpjq_ctx *ctx, uint32_t slot, const uint8_t *d_bytes, uint64_t n_bytes,
+                                       char *err, size_t err_len) {
+  if (!ctx || slot >= ctx->n_slots || (!d_bytes && n_bytes)) return WARPJQ_ERR_INVALID_ARG;
+  if (n_bytes > MAX_CHUNK || (ctx->rows && n_bytes > ctx->chunk_cap)) return WARPJQ_ERR_INVALID_ARG;
+  Slot &s = ctx->slots[slot];
+  s.n_bytes = n_bytes;
+  s.resident = true;
+  s.d_src = d_bytes;
+  HIP_TRY(hipEventRecord(s.ev0, s.stream), "hipEventRecord");
+  HIP_TRY(hipEventRecord(s.ev1, s.stream), "hipEventRecord");
+  return launch_chain(ctx, s, d_bytes, n_bytes, err, err_len);
+}
+
+warpjq_status warpjq_wait(warpjq_ctx *ctx, uint32_t slot, warpjq_chunk_result *out, char *err, size_t err_len) {
+  if (!ctx || slot >= ctx->n_slots || !out) return WARPJQ_ERR_INVALID_ARG;
+  Slot &s = ctx->slots[slot];
+  HIP_TRY(hipStreamSynchronize(s.stream), "stream sync");
+  HIP_TRY(hipGetLastError(), "kernel");
+  memset(out, 0, sizeof *out);
+  out->first_invalid = 0xFFFFFFFFu;
+  if (!s.submitted || s.n_bytes == 0) return WARPJQ_OK;
+  const DevCtr c = *s.h_ctr;
+  out->n_lines = c.n_lines;
+  out->n_blank = c.n_blank;
+  out->n_invalid = c.n_invalid;
+  out->n_type_error = c.n_type_error;
+  out->n_fallback = c.n_fallback;
+  out->first_invalid = c.first_invalid;
+  out->chunk_overflow = c.overflow || c.fb_count > ctx->fb_cap;
+  out->agg.saw_non_numeric = c.saw_non_numeric;
+  if (out->chunk_overflow) {
+    out->n_lines = 0;
+    return WARPJQ_OK;
+  }
+  uint32_t nfb = c.fb_count;
+  if (nfb) {
+    HIP_TRY(hipMemcpyAsync(s.h_fb_idx, s.d_fb_idx, nfb * 4ull, hipMemcpyDeviceToHost, s.stream), "D2H(fallback)");
+    HIP_TRY(hipMemcpyAsync(s.h_fb_off, s.d_fb_off, nfb * 4ull, hipMemcpyDeviceToHost, s.stream), "D2H(fallback)");
+    HIP_TRY(hipMemcpyAsync(s.h_fb_len, s.d_fb_len, nfb * 4ull, hipMemcpyDeviceToHost, s.stream), "D2H(fallback)");
+  }
+  out->fallback_idx = s.h_fb_idx;
+  out->fallback_off = s.h_fb_off;
+  out->fallback_len = s.h_fb_len;
+  out->n_fallback = nfb;
+  if (ctx->dprog.output_kind == WARPJQ_OUT_AGG) {
+    if (ctx->grouped) {
+      uint32_t ng = std::min<uint32_t>(c.n_groups, ctx->group_cap);
+      if (ng) {
+        HIP_TRY(hipMemcpyAsync(s.h_groups, s.d_groups, (size_t)ng * sizeof(warpjq_group), hipMemcpyDeviceToHost,
+                               s.stream),
+                "D2H(groups)");
+        HIP_TRY(hipMemcpyAsync(s.h_gext, s.d_gext, (size_t)ng * sizeof(warpjq_amd_agg_ext), hipMemcpyDeviceToHost,
+                               s.stream),
+                "D2H(groups)");
+      }
+      HIP_TRY(hipStreamSynchronize(s.stream), "sync(groups)");
+      for (uint32_t i = 0; i < ng; i++) s.h_groups[i].sum = warpjq_amd_ext_value(&s.h_gext[i]);
+      out->groups = s.h_groups;
+      out->n_groups = ng;
+      out->group_overflow = c.group_overflow;
+    } else {
+      HIP_TRY(hipStreamSynchronize(s.stream), "sync(agg)");
+      out->agg = s.h_aggout->agg;
+      out->agg.sum = warpjq_amd_ext_value(&s.h_aggout->ext);
+      out->agg.saw_non_numeric = c.saw_non_numeric;
+    }
+    return WARPJQ_OK;
+  }
+  uint64_t nsel = c.n_sel;
+  out->n_selected = nsel;
+  out->out_bytes = s.h_out;
+  out->out_line_idx = s.h_sel_idx;
+  out->out_row_off = s.h_row_off;
+  if (nsel) {
+    HIP_TRY(hipMemcpyAsync(s.h_row_off, s.d_row_off, (nsel + 1) * 8, hipMemcpyDeviceToHost, s.stream), "D2H(rows)");
+    HIP_TRY(hipMemcpyAsync(s.h_sel_idx, s.d_sel_idx, nsel * 4, hipMemcpyDeviceToHost, s.stream), "D2H(rows)");
+    HIP_TRY(hipMemcpyAsync(s.h_out, s.d_out, c.out_total, hipMemcpyDeviceToHost, s.stream), "D2H(rows)");
+    out->out_len = c.out_total;
+  }
+  HIP_TRY(hipStreamSynchronize(s.stream), "sync(rows)");
+  return WARPJQ_OK;
+}
+
+double warpjq_amd_ext_value(const warpjq_amd_agg_ext *e) {
+  // exact 128-bit integer part, rounded once, plus the compensated float part
+  __int128 i = (__int128)(((unsigned __int128)e->isum_hi << 64) | e->isum_lo);
+  double hi = (double)i;
+  __int128 rem = i - (__int128)hi;
+  double s = hi + e->fsum;
+  double bp = s - hi;
+  double err = (hi - (s - bp)) + (e->fsum - bp);
+  return s + (err + (double)rem + e->fcomp);
+}
+
+warpjq_status warpjq_amd_wait_ext(warpjq_ctx *ctx, uint32_t slot, warpjq_amd_agg_ext *agg_ext,
+                                  const warpjq_amd_agg_ext **group_ext) {
+  if (!ctx || slot >= ctx->n_slots) return WARPJQ_ERR_INVALID_ARG;
+  Slot &s = ctx->slots[slot];
+  if (agg_ext) {
+    if (ctx->dprog.output_kind == WARPJQ_OUT_AGG && !ctx->grouped && s.n_bytes) *agg_ext = s.h_aggout->ext;
+    else memset(agg_ext, 0, sizeof *agg_ext);
+  }
+  if (group_ext) *group_ext = ctx->grouped ? s.h_gext : nullptr;
+  return WARPJQ_OK;
+}
+
+warpjq_status warpjq_amd_slot_timing(warpjq_ctx *ctx, uint32_t slot, float *h2d_ms, float *kernel_ms) {
+  if (!ctx || slot >= ctx->n_slots) return WARPJQ_ERR_INVALID_ARG;
+  Slot &s = ctx->slots[slot];
+  float a = 0, b = 0;
+  if (hipEventElapsedTime(&a, s.ev0, s.ev1) != hipSuccess) a = 0;
+  if (hipEventElapsedTime(&b, s.ev1, s.ev2) != hipSuccess) b = 0;
+  if (h2d_ms) *h2d_ms = a;
+  if (kernel_ms) *kernel_ms = b;
+  return WARPJQ_OK;
+}
+
+uint32_t warpjq_amd_kernel_width(const warpjq_ctx *ctx) { return ctx ? ctx->host.ns : 0; }
+
+}  // extern "C"
