@@ -1,0 +1,188 @@
+"""Benchmark: the 8 GiB nginx-schema filter-count on one MI355X.
+
+One step = one pass of `select(.status == 500) | count` over 8 GiB of
+nginx-schema NDJSON already resident in HBM (the chunk split in <=4 GiB
+pieces, each through warpjq_amd_submit_device). `value` is input GB/s over
+the timed steps. Also reported, once each after the timed loop: the other
+three queries of the same workload (kernel-only GB/s), the end-to-end
+host-buffer run (pinned H2D + kernels), the roofline of the main kernel
+(hipEvents around it, on its own stream) and the CPU baseline (the oracle on
+a bounded sample, host threads).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+QUERY = "select(.status == 500) | count"
+OTHER = ["select(.status >= 500) | {p: .path, b: .bytes}", "group_by(.host) | count", "select(.status == 500)"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def split_points(host, n, piece):
+    """Newline-aligned piece boundaries of at most `piece` bytes."""
+    pts = [0]
+    while n - pts[-1] > piece:
+        target = pts[-1] + piece
+        j = target - 1
+        while host[j] != 10:
+            j -= 1
+        pts.append(j + 1)
+    pts.append(n)
+    return list(zip(pts[:-1], pts[1:]))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size-gib", type=float, default=8.0)
+    ap.add_argument("--piece-mib", type=int, default=4095)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-extras", action="store_true", help="only the timed filter-count loop")
+    ap.add_argument("--cpu-sample-mib", type=int, default=1024)
+    ap.add_argument("--claim-kib", type=int, default=0, help="bytes per wave claim (tuning)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
+              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import warpjq_amd as w
+
+    w.probe()
+    dev = w.device_name()
+    target = int(args.size_gib * (1 << 30))
+    t0 = time.time()
+    host, n, n_lines, hist = w.gen_nginx(target, seed=args.seed, threads=min(16, os.cpu_count() or 1))
+    log(f"generated {n} bytes, {n_lines} lines in {time.time() - t0:.1f}s on {dev}")
+
+    d = torch.from_numpy(host).to("cuda")  # n + 256 bytes of padding
+    torch.cuda.synchronize()
+    base = d.data_ptr()
+    pieces = split_points(host, n, args.piece_mib << 20)
+    ctx = w.Context(QUERY, chunk_bytes=64 << 20, n_slots=1)
+    if args.claim_kib:
+        ctx.set_claim(args.claim_kib << 10)
+
+    def step():
+        # one stream: pieces run back to back, so per-kernel events are clean
+        total = 0
+        main_ms = 0.0
+        for a, b in pieces:
+            ctx.submit_device(0, base + a, b - a)
+            r = ctx.wait(0)
+            total += r.agg.count
+            main_ms += ctx.timing(0)[2]
+        return total, main_ms
+
+    for _ in range(args.warmup):
+        step()
+
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    mains = []
+    for _ in range(args.steps):
+        count, m = step()
+        mains.append(m)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    if count != hist[500]:  # a size-independent check: the generator counted its own 500s
+        raise SystemExit(f"parity: device count {count} != generator count {hist[500]}")
+    ms_per_step = wall * 1e3 / args.steps
+    gbps = n / (ms_per_step / 1e3) / 1e9
+    main_ms = sum(mains) / len(mains)
+    achieved = n / (main_ms / 1e3) / 1e9  # algorithmic bytes (input) / main-kernel time, per step
+
+    extras = {}
+    if not args.no_extras:
+        # the rest of the workload's queries, kernel-only, one pass each
+        for q in OTHER:
+            c2 = w.Context(q, chunk_bytes=512 << 20, n_slots=1)
+            ps = split_points(host, n, 512 << 20)
+            chain = 0.0
+            rows = 0
+            out_bytes = 0
+            for a, b in ps:
+                c2.submit_device(0, base + a, b - a)
+                r = c2.wait(0)
+                rows += r.n_selected + r.n_groups
+                out_bytes += r.out_len
+                chain += c2.timing(0)[1]
+            c2.close()
+            extras[q] = {"kernel_gbps": round(n / (chain / 1e3) / 1e9, 1), "rows": rows,
+                         "alg_gbps": round((n + out_bytes) / (chain / 1e3) / 1e9, 1)}
+            log(q, extras[q])
+        # end to end from a host buffer (pinned staging + H2D + kernels + D2H)
+        t = time.perf_counter()
+        out, st = w.run(QUERY, host, chunk_bytes=64 << 20, n=n)
+        e2e = time.perf_counter() - t
+        assert int(out) == hist[500]
+        extras["e2e"] = {"gbps": round(n / e2e / 1e9, 2), "h2d_ms": round(st["h2d_ms"], 1),
+                         "kernel_ms": round(st["kernel_ms"], 1), "chunks": st["chunks"],
+                         "h2d_gbps": round(n / (st["h2d_ms"] / 1e3) / 1e9, 1) if st["h2d_ms"] else None}
+        log("e2e", extras["e2e"])
+
+    # CPU baseline: the oracle on a bounded sample of the same input
+    cpu = None
+    try:
+        import oracle
+        cut = min(n, args.cpu_sample_mib << 20)
+        while cut < n and host[cut - 1] != 10:
+            cut -= 1
+        sample = host[:cut].tobytes()
+        threads = min(16, os.cpu_count() or 1)
+        best = None
+        for _ in range(2):
+            t = time.perf_counter()
+            out, _ = oracle.run(QUERY, sample, threads=threads, chunk_bytes=256 << 20)
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        ctx_s = w.Context(QUERY, chunk_bytes=64 << 20, n_slots=1)
+        ctx_s.submit_device(0, base, cut)
+        assert ctx_s.wait(0).agg.count == int(out), "GPU and oracle disagree on the sample"
+        ctx_s.close()
+        cpu = {"value": round(cut / best / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+               "sample": f"first {cut >> 20} MiB ({cut} B) of the same input, {QUERY}, best of 2"}
+    except Exception as e:  # the baseline is informative only
+        log("cpu baseline skipped:", e)
+
+    line = {
+        "metric": "input GB/s (kernel) of select(.status == 500) | count on 8 GiB nginx NDJSON resident in HBM",
+        "value": round(gbps, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "nginx NDJSON, select(.status == 500) | count, resident in HBM",
+                   "bytes": n, "lines": n_lines, "avg_line_bytes": round(n / n_lines, 1),
+                   "pieces": len(pieces), "device": dev},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "kernel": "k_main", "main_ms_per_step": round(main_ms, 3)},
+        "cpu_baseline": cpu,
+        "extras": extras,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -44,12 +44,18 @@ warpjq_status warpjq_amd_wait_ext(warpjq_ctx *ctx, uint32_t slot, warpjq_amd_agg
 warpjq_status warpjq_amd_submit_device(warpjq_ctx *ctx, uint32_t slot, const uint8_t *d_bytes,
                                        uint64_t n_bytes, char *err, size_t err_len);
 
-/* Device time of the last submit on `slot`: the H2D copy and the kernel
- * chain (hipEvent). Valid after warpjq_wait. No counterpart. */
-warpjq_status warpjq_amd_slot_timing(warpjq_ctx *ctx, uint32_t slot, float *h2d_ms, float *kernel_ms);
+/* Device time of the last submit on `slot` (hipEvent on the slot's stream):
+ * the H2D copy, the whole kernel chain, and the one-pass main kernel alone.
+ * Valid after warpjq_wait. No counterpart in the reference. */
+warpjq_status warpjq_amd_slot_timing(warpjq_ctx *ctx, uint32_t slot, float *h2d_ms, float *kernel_ms,
+                                     float *main_ms);
 
 /* Slot-table width of the kernel instantiation a context uses (1..16). */
 uint32_t warpjq_amd_kernel_width(const warpjq_ctx *ctx);
+
+/* Bytes each wave claims per ticket in the one-pass kernel (1 KiB steps,
+ * 1-64 KiB; default 32 KiB). A tuning knob; results do not depend on it. */
+warpjq_status warpjq_amd_set_claim(warpjq_ctx *ctx, uint32_t claim_bytes);
 
 /* What one driver run did (the fields of RunStats, exec/mod.rs:65-88, plus
  * device time). */
@@ -83,6 +89,15 @@ typedef struct warpjq_amd_program warpjq_amd_program;
 warpjq_amd_program *warpjq_amd_compile(const char *query, int csv, char *err, size_t err_len);
 const warpjq_program *warpjq_amd_program_view(const warpjq_amd_program *p);
 void warpjq_amd_program_free(warpjq_amd_program *p);
+
+/* Seeded synthetic nginx-schema NDJSON (ts, host, method, path, status,
+ * bytes, ua; ~220 B/line) for benchmarks: whole lines up to target_bytes,
+ * generated on `threads` host threads; `pad` zero bytes follow the data.
+ * status_hist[8] counts 200, 404, 500, 301, 304, 403, 502, 503. Free *out
+ * with warpjq_amd_free. No counterpart in the reference (its generator,
+ * crates/warpjq-core/src/gen.rs, has a different schema). */
+int warpjq_amd_gen_nginx(uint64_t seed, uint64_t target_bytes, uint32_t threads, uint64_t pad, uint8_t **out,
+                         uint64_t *n_bytes, uint64_t *n_lines, uint64_t *status_hist);
 
 #ifdef __cplusplus
 }

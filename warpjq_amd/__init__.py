@@ -90,9 +90,9 @@ REFERENCE_SYMBOLS = ["warpjq_probe", "warpjq_abi_check", "warpjq_ctx_create", "w
                      "warpjq_slot_buffer", "warpjq_slot_capacity", "warpjq_max_lines", "warpjq_submit",
                      "warpjq_wait", "warpjq_device_name"]
 ADDITIVE_SYMBOLS = ["warpjq_amd_ext_value", "warpjq_amd_wait_ext", "warpjq_amd_submit_device",
-                    "warpjq_amd_slot_timing", "warpjq_amd_kernel_width", "warpjq_amd_run_buffer",
+                    "warpjq_amd_slot_timing", "warpjq_amd_kernel_width", "warpjq_amd_set_claim", "warpjq_amd_run_buffer",
                     "warpjq_amd_free", "warpjq_amd_compile", "warpjq_amd_program_view",
-                    "warpjq_amd_program_free"]
+                    "warpjq_amd_program_free", "warpjq_amd_gen_nginx"]
 
 
 def build(jobs=8):
@@ -128,10 +128,12 @@ def lib():
         L.warpjq_amd_ext_value.restype = f64
         L.warpjq_amd_wait_ext.argtypes = [ctypes.c_void_p, u32, P(AggExt), P(P(AggExt))]
         L.warpjq_amd_submit_device.argtypes = [ctypes.c_void_p, u32, ctypes.c_void_p, u64, cp, sz]
-        L.warpjq_amd_slot_timing.argtypes = [ctypes.c_void_p, u32, P(ctypes.c_float), P(ctypes.c_float)]
+        L.warpjq_amd_slot_timing.argtypes = [ctypes.c_void_p, u32, P(ctypes.c_float), P(ctypes.c_float),
+                                             P(ctypes.c_float)]
         L.warpjq_amd_kernel_width.argtypes = [ctypes.c_void_p]
         L.warpjq_amd_kernel_width.restype = u32
-        L.warpjq_amd_run_buffer.argtypes = [cp, cp, u64, ctypes.c_int, ctypes.c_int, u64, u64,
+        L.warpjq_amd_set_claim.argtypes = [ctypes.c_void_p, u32]
+        L.warpjq_amd_run_buffer.argtypes = [cp, ctypes.c_void_p, u64, ctypes.c_int, ctypes.c_int, u64, u64,
                                             P(P(ctypes.c_uint8)), P(u64), P(Stats), cp, sz]
         L.warpjq_amd_run_buffer.restype = ctypes.c_int
         L.warpjq_amd_free.argtypes = [ctypes.c_void_p]
@@ -142,6 +144,8 @@ def lib():
         L.warpjq_amd_program_view.restype = P(Program)
         L.warpjq_amd_program_free.argtypes = [ctypes.c_void_p]
         L.warpjq_amd_program_free.restype = None
+        L.warpjq_amd_gen_nginx.argtypes = [u64, u64, u32, u64, P(P(ctypes.c_uint8)), P(u64), P(u64), P(u64)]
+        L.warpjq_amd_gen_nginx.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -184,16 +188,30 @@ def abi_check(sizes=None):
     return lib().warpjq_abi_check(s, 6)
 
 
-def run(query, data, fmt=FMT_NDJSON, on_invalid=SKIP, chunk_bytes=GPU_CHUNK, max_line=MAX_LINE):
-    """Runs `query` over `data` on the GPU. Returns (output bytes, stats dict)."""
+def _addr(data, n=None):
+    """(address, length) of bytes, bytearray or a numpy uint8 array."""
+    if isinstance(data, bytes):
+        return ctypes.cast(ctypes.c_char_p(data), ctypes.c_void_p).value, len(data) if n is None else n
+    if hasattr(data, "ctypes"):
+        return data.ctypes.data, data.nbytes if n is None else n
+    buf = (ctypes.c_char * len(data)).from_buffer(data)
+    return ctypes.addressof(buf), len(data) if n is None else n
+
+
+def run(query, data, fmt=FMT_NDJSON, on_invalid=SKIP, chunk_bytes=GPU_CHUNK, max_line=MAX_LINE, n=None):
+    """Runs `query` over `data` (bytes, or a numpy uint8 array whose first
+    `n` bytes are the input) on the GPU. Returns (output bytes, stats)."""
     L = lib()
+    keep = data
+    addr, nbytes = _addr(data, n)
     out = P(ctypes.c_uint8)()
     out_len = u64()
     st = Stats()
     e = ctypes.create_string_buffer(2048)
     q = query.encode() if isinstance(query, str) else query
-    rc = L.warpjq_amd_run_buffer(q, data, len(data), fmt, on_invalid, chunk_bytes, max_line, ctypes.byref(out),
+    rc = L.warpjq_amd_run_buffer(q, addr, nbytes, fmt, on_invalid, chunk_bytes, max_line, ctypes.byref(out),
                                  ctypes.byref(out_len), ctypes.byref(st), e, 2048)
+    del keep
     stats = {k: getattr(st, k) for k, _ in Stats._fields_}
     if rc != 0:
         raise RunError(rc, st.err_line, e.value.decode(errors="replace"))
@@ -238,6 +256,10 @@ class Context:
     def __del__(self):
         self.close()
 
+    def set_claim(self, nbytes):
+        if lib().warpjq_amd_set_claim(self.ctx, nbytes) != 0:
+            raise ValueError("claim must be 1-64 KiB in 1 KiB steps")
+
     @property
     def capacity(self):
         return lib().warpjq_slot_capacity(self.ctx)
@@ -277,9 +299,32 @@ class Context:
         return x, g
 
     def timing(self, slot):
-        a, b = ctypes.c_float(), ctypes.c_float()
-        lib().warpjq_amd_slot_timing(self.ctx, slot, ctypes.byref(a), ctypes.byref(b))
-        return a.value, b.value
+        """(h2d_ms, chain_ms, main_kernel_ms) of the last submit on `slot`."""
+        a, b, m = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        lib().warpjq_amd_slot_timing(self.ctx, slot, ctypes.byref(a), ctypes.byref(b), ctypes.byref(m))
+        return a.value, b.value, m.value
+
+
+STATUS_ORDER = (200, 404, 500, 301, 304, 403, 502, 503)
+
+
+def gen_nginx(target_bytes, seed=1, threads=None, pad=256):
+    """Synthetic nginx-schema NDJSON. Returns (numpy uint8 array of n+pad
+    bytes, n_bytes, n_lines, {status: count})."""
+    import numpy as np
+    threads = threads or min(16, os.cpu_count() or 1)
+    out = P(ctypes.c_uint8)()
+    n, lines = u64(), u64()
+    hist = (u64 * 8)()
+    rc = lib().warpjq_amd_gen_nginx(seed, target_bytes, threads, pad, ctypes.byref(out), ctypes.byref(n),
+                                    ctypes.byref(lines), hist)
+    if rc != 0:
+        raise MemoryError("generator allocation failed")
+    total = n.value + pad
+    arr = np.empty(total, dtype=np.uint8)
+    ctypes.memmove(arr.ctypes.data, out, total)
+    lib().warpjq_amd_free(out)
+    return arr, n.value, lines.value, dict(zip(STATUS_ORDER, list(hist)))
 
 
 def result_dict(r):

@@ -302,7 +302,7 @@ static int drain(Run &r, uint32_t slot, const Chunk &c, std::vector<Chunk> &redo
     return 3;
   }
   float h2d = 0, kms = 0;
-  warpjq_amd_slot_timing(r.ctx, slot, &h2d, &kms);
+  warpjq_amd_slot_timing(r.ctx, slot, &h2d, &kms, nullptr);
   r.st.h2d_ms += h2d;
   r.st.kernel_ms += kms;
   r.st.chunks++;

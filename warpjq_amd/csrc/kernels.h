@@ -47,9 +47,11 @@ using namespace wjqd;
 // tunables
 // ---------------------------------------------------------------------------
 
-#define TILE_BYTES 16384u            // bytes per tile (16 steps of 1 KiB)
+#define CLAIM_MIN 1024u              // smallest claim a wave takes (one scan step)
+#define CLAIM_MAX 65536u             // largest claim (the queue index is 16-bit safe)
 #define STEP_BYTES 1024u             // 64 lanes x 16 B
-#define QCAP 1024u                   // per-wave LDS queue of line starts
+#define STAGE_BYTES 16384u           // LDS staging of one round's line span
+#define QCAP 512u                    // per-wave LDS queue of line starts
 #define QMASK (QCAP - 1u)
 #define LGROUP 64u                   // per-wave LDS group table entries
 #define GROUP_MAX_PROBE 128u
@@ -89,6 +91,7 @@ struct KArgs {
   const uint8_t *d;
   uint32_t n;
   uint32_t ntiles;
+  uint32_t claim;  // bytes per claimed tile (multiple of STEP_BYTES)
   DProg P;
   DevCtr *ctr;
   unsigned long long *tile_state;
@@ -108,18 +111,24 @@ struct KArgs {
 // ---------------------------------------------------------------------------
 
 // Caches one 16-byte block per lane; positions at or past n read as '\n'.
+// Blocks inside the round's staged span [slo, shi) come from LDS, the rest
+// from global memory (L2).
 struct DevReader {
   const uint8_t *d;
   uint32_t n;
   uint32_t cb;
   uint32_t v0, v1, v2, v3;
+  const uint4 *stg;
+  uint32_t slo, shi;
 
-  __device__ DevReader(const uint8_t *d_, uint32_t n_) : d(d_), n(n_), cb(0xFFFFFFFFu), v0(0), v1(0), v2(0), v3(0) {}
+  __device__ DevReader(const uint8_t *d_, uint32_t n_)
+      : d(d_), n(n_), cb(0xFFFFFFFFu), v0(0), v1(0), v2(0), v3(0), stg(nullptr), slo(0), shi(0) {}
 
   __device__ __forceinline__ uint32_t dword(uint32_t k) {
     uint32_t b = k >> 2;
     if (b != cb) {
-      uint4 v = *((const uint4 *)d + b);
+      uint32_t off = (b << 4) - slo;
+      uint4 v = off < shi - slo ? stg[off >> 4] : *((const uint4 *)d + b);
       v0 = v.x;
       v1 = v.y;
       v2 = v.z;
@@ -413,6 +422,7 @@ template <int NS, bool GRP>
 __global__ __launch_bounds__(64) void k_main(KArgs a) {
   __shared__ uint32_t qs[QCAP];
   __shared__ uint32_t qi[QCAP];
+  __shared__ uint4 stage[STAGE_BYTES / 16];
   LGroup *lgp = LGroupStore<GRP>::get();
 
   const uint32_t lane = lane_id();
@@ -450,10 +460,14 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
   bool tiles_left = true;
 
   DevReader rd(a.d, a.n);
+  rd.stg = stage;
+  uint32_t claim_end = 0;
 
   for (;;) {
-    // ---- refill the queue until a full round is available ----
-    while (qlen < 64 && (pend || tiles_left)) {
+    // ---- refill: rounds never mix claims, so a round's lines are one
+    // contiguous span of the chunk ----
+    while (qlen == 0 && (pend || tiles_left)) {
+      qhead = 0;
       uint32_t t, step0;
       uint64_t idx0 = 0;
       bool known_base;
@@ -474,7 +488,7 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
         step0 = 0;
         known_base = false;
       }
-      const uint32_t t0 = t * TILE_BYTES;
+      const uint32_t t0 = t * a.claim;
       const uint32_t tq0 = qhead + qlen;  // first queue entry of this tile
       uint32_t local = 0;                 // line starts seen in this tile from step0
       uint32_t stored = 0;
@@ -486,7 +500,7 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
         if (p0 > 0) c = a.d[p0 - 1] == '\n' ? 1 : 0;
         carry = c;
       }
-      for (uint32_t step = step0; step < TILE_BYTES / STEP_BYTES; step++) {
+      for (uint32_t step = step0; step < a.claim / STEP_BYTES; step++) {
         uint32_t p = t0 + step * STEP_BYTES + lane * 16;
         if (t0 + step * STEP_BYTES >= a.n) break;
         uint32_t m = 0;
@@ -546,12 +560,26 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
         pend_idx += idx0;
       }
       qlen += stored;
+      claim_end = t0 + a.claim;
       __syncthreads();
     }
     if (qlen == 0) break;
 
     // ---- one round: lane k evaluates queued line k ----
     uint32_t take = qlen < 64 ? qlen : 64;
+    {
+      // stage the round's span [lo, hi) into LDS with coalesced 16-byte loads
+      uint32_t lo = qs[qhead & QMASK] & ~15u;
+      uint32_t hi = qlen > take ? qs[(qhead + take) & QMASK] : claim_end + 1024u;
+      uint32_t cap_hi = (a.n + 16u) & ~15u;
+      if (hi > cap_hi) hi = cap_hi;
+      if (hi > lo + STAGE_BYTES) hi = lo + STAGE_BYTES;
+      hi = (hi + 15u) & ~15u;
+      for (uint32_t off = lane * 16u; lo + off < hi; off += 1024u) stage[off >> 4] = *((const uint4 *)(a.d + lo + off));
+      rd.slo = lo;
+      rd.shi = hi;
+      __syncthreads();
+    }
     if (lane < take) {
       uint32_t e = (qhead + lane) & QMASK;
       uint32_t s = qs[e];
@@ -654,6 +682,7 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
     }
     qhead = (qhead + take) & QMASK;
     qlen -= take;
+    __syncthreads();
   }
 
   // ---- flush ----

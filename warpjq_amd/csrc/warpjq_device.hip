@@ -41,35 +41,68 @@ struct AggOut {
   warpjq_amd_agg_ext ext;
 };
 
-__global__ void k_agg_final(const AggPart *parts, uint32_t nparts, const DevCtr *ctr, AggOut *out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// 256 threads: each folds a strided subset in order, then a fixed tree, so
+// the result is deterministic for a given grid.
+__global__ __launch_bounds__(256) void k_agg_final(const AggPart *parts, uint32_t nparts, const DevCtr *ctr,
+                                                    AggOut *out) {
+  __shared__ unsigned long long s_cnt[256], s_num[256], s_lo[256], s_hi[256];
+  __shared__ double s_s[256], s_c[256], s_mn[256], s_mx[256];
+  const uint32_t t = threadIdx.x;
   unsigned long long cnt = 0, num = 0, lo = 0, hi = 0;
   double s = 0.0, c = 0.0, mn = INFINITY, mx = -INFINITY;
-  for (uint32_t i = 0; i < nparts; i++) {
+  for (uint32_t i = t; i < nparts; i += 256) {
     const AggPart &p = parts[i];
     cnt += p.count;
     num += p.numeric;
     unsigned long long nlo = lo + p.isum_lo;
     hi += p.isum_hi + (nlo < lo ? 1 : 0);
     lo = nlo;
-    double t = s + p.fsum;
-    double bp = t - s;
-    c += (s - (t - bp)) + (p.fsum - bp) + p.fcomp;
-    s = t;
+    double x = s + p.fsum;
+    double bp = x - s;
+    c += (s - (x - bp)) + (p.fsum - bp) + p.fcomp;
+    s = x;
     if (p.mn < mn) mn = p.mn;
     if (p.mx > mx) mx = p.mx;
   }
-  out->ext.isum_lo = lo;
-  out->ext.isum_hi = hi;
-  out->ext.fsum = s;
-  out->ext.fcomp = c;
-  out->agg.count = cnt;
-  out->agg.numeric = num;
-  out->agg.sum = 0.0;  // filled on the host from the exact parts
-  out->agg.min = mn;
-  out->agg.max = mx;
-  out->agg.saw_non_numeric = ctr->saw_non_numeric;
-  out->agg._pad = 0;
+  s_cnt[t] = cnt;
+  s_num[t] = num;
+  s_lo[t] = lo;
+  s_hi[t] = hi;
+  s_s[t] = s;
+  s_c[t] = c;
+  s_mn[t] = mn;
+  s_mx[t] = mx;
+  __syncthreads();
+  for (uint32_t w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      uint32_t o = t + w;
+      s_cnt[t] += s_cnt[o];
+      s_num[t] += s_num[o];
+      unsigned long long nlo = s_lo[t] + s_lo[o];
+      s_hi[t] += s_hi[o] + (nlo < s_lo[t] ? 1 : 0);
+      s_lo[t] = nlo;
+      double x = s_s[t] + s_s[o];
+      double bp = x - s_s[t];
+      s_c[t] += (s_s[t] - (x - bp)) + (s_s[o] - bp) + s_c[o];
+      s_s[t] = x;
+      if (s_mn[o] < s_mn[t]) s_mn[t] = s_mn[o];
+      if (s_mx[o] > s_mx[t]) s_mx[t] = s_mx[o];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out->ext.isum_lo = s_lo[0];
+    out->ext.isum_hi = s_hi[0];
+    out->ext.fsum = s_s[0];
+    out->ext.fcomp = s_c[0];
+    out->agg.count = s_cnt[0];
+    out->agg.numeric = s_num[0];
+    out->agg.sum = 0.0;  // filled on the host from the exact parts
+    out->agg.min = s_mn[0];
+    out->agg.max = s_mx[0];
+    out->agg.saw_non_numeric = ctr->saw_non_numeric;
+    out->agg._pad = 0;
+  }
 }
 
 __global__ void k_group_compact(GroupTab g, DevCtr *ctr, warpjq_group *out, warpjq_amd_agg_ext *ext) {
@@ -268,6 +301,7 @@ namespace {
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;  // before H2D, after H2D, after kernels
+  hipEvent_t evm0 = nullptr, evm1 = nullptr;              // around the main kernel
   uint8_t *h_pinned = nullptr;
   uint8_t *d_data = nullptr;
   const uint8_t *d_src = nullptr;  // what the last submit read (staging or a resident buffer)
@@ -313,6 +347,7 @@ struct warpjq_ctx {
   uint32_t group_cap = 0;
   uint32_t grid = 0;
   uint32_t max_tiles = 0;
+  uint32_t claim = 32768;
   bool grouped = false, rows = false;
   int device = 0;
 };
@@ -384,6 +419,8 @@ static void free_slot(Slot &s) {
   if (s.ev0) (void)hipEventDestroy(s.ev0);
   if (s.ev1) (void)hipEventDestroy(s.ev1);
   if (s.ev2) (void)hipEventDestroy(s.ev2);
+  if (s.evm0) (void)hipEventDestroy(s.evm0);
+  if (s.evm1) (void)hipEventDestroy(s.evm1);
   if (s.stream) (void)hipStreamDestroy(s.stream);
   s = Slot{};
 }
@@ -393,6 +430,8 @@ static warpjq_status alloc_slot(warpjq_ctx *ctx, Slot &s, char *err, size_t err_
   HIP_TRY(hipEventCreate(&s.ev0), "hipEventCreate");
   HIP_TRY(hipEventCreate(&s.ev1), "hipEventCreate");
   HIP_TRY(hipEventCreate(&s.ev2), "hipEventCreate");
+  HIP_TRY(hipEventCreate(&s.evm0), "hipEventCreate");
+  HIP_TRY(hipEventCreate(&s.evm1), "hipEventCreate");
   HIP_TRY(halloc(&s.h_pinned, ctx->chunk_cap), "hipHostMalloc(staging)");
   HIP_TRY(dalloc(&s.d_data, ctx->chunk_cap + DATA_PAD), "hipMalloc(data)");
   HIP_TRY(hipMemsetAsync(s.d_data, 0, ctx->chunk_cap + DATA_PAD, s.stream), "hipMemset(data)");
@@ -529,7 +568,7 @@ warpjq_status warpjq_ctx_create(const warpjq_program *prog, uint64_t max_chunk_b
   ctx->group_cap = (uint32_t)g;
   ctx->grid = g_num_cus * 16;
   // tiles for the largest chunk a slot or a resident buffer may hold
-  ctx->max_tiles = (uint32_t)((MAX_CHUNK + TILE_BYTES - 1) / TILE_BYTES) + 1;
+  ctx->max_tiles = (uint32_t)((MAX_CHUNK + CLAIM_MIN - 1) / CLAIM_MIN) + 1;
 
   // one device allocation holding every program table
   std::vector<uint8_t> buf;
@@ -604,7 +643,8 @@ uint64_t warpjq_max_lines(const warpjq_ctx *ctx) { return ctx ? ctx->max_lines :
 static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const uint8_t *src, uint64_t n, char *err,
                                   size_t err_len) {
   hipStream_t st = s.stream;
-  const uint32_t ntiles = (uint32_t)((n + TILE_BYTES - 1) / TILE_BYTES);
+  const uint32_t claim = ctx->claim;
+  const uint32_t ntiles = (uint32_t)((n + claim - 1) / claim);
   hipLaunchKernelGGL(k_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (ntiles + 255) / 256))), dim3(256),
                      0, st, s.d_ctr, s.d_tile_state, ntiles);
   if (n > 0) {
@@ -612,6 +652,7 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const uint8_t *src, 
     a.d = src;
     a.n = (uint32_t)n;
     a.ntiles = ntiles;
+    a.claim = claim;
     a.P = ctx->dprog;
     a.ctr = s.d_ctr;
     a.tile_state = s.d_tile_state;
@@ -625,6 +666,7 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const uint8_t *src, 
     a.fb_len = s.d_fb_len;
     a.fb_cap = ctx->fb_cap;
     uint32_t grid = std::min<uint32_t>(ctx->grid, ntiles);
+    HIP_TRY(hipEventRecord(s.evm0, st), "hipEventRecord");
     switch (ctx->host.ns) {
       case 1: launch_main_1(ctx->grouped, grid, st, a); break;
       case 2: launch_main_2(ctx->grouped, grid, st, a); break;
@@ -633,11 +675,12 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const uint8_t *src, 
       default: launch_main_16(ctx->grouped, grid, st, a); break;
     }
     HIP_TRY(hipGetLastError(), "k_main");
+    HIP_TRY(hipEventRecord(s.evm1, st), "hipEventRecord");
     if (ctx->dprog.output_kind == WARPJQ_OUT_AGG) {
       if (ctx->grouped) {
         hipLaunchKernelGGL(k_group_compact, dim3(1024), dim3(256), 0, st, s.gt, s.d_ctr, s.d_groups, s.d_gext);
       } else {
-        hipLaunchKernelGGL(k_agg_final, dim3(1), dim3(64), 0, st, s.d_parts, grid, s.d_ctr, s.d_aggout);
+        hipLaunchKernelGGL(k_agg_final, dim3(1), dim3(256), 0, st, s.d_parts, grid, s.d_ctr, s.d_aggout);
       }
     } else {
       uint64_t lines_bound = std::min<uint64_t>(ctx->max_lines, n + 1);
@@ -790,17 +833,26 @@ warpjq_status warpjq_amd_wait_ext(warpjq_ctx *ctx, uint32_t slot, warpjq_amd_agg
   return WARPJQ_OK;
 }
 
-warpjq_status warpjq_amd_slot_timing(warpjq_ctx *ctx, uint32_t slot, float *h2d_ms, float *kernel_ms) {
+warpjq_status warpjq_amd_slot_timing(warpjq_ctx *ctx, uint32_t slot, float *h2d_ms, float *kernel_ms,
+                                     float *main_ms) {
   if (!ctx || slot >= ctx->n_slots) return WARPJQ_ERR_INVALID_ARG;
   Slot &s = ctx->slots[slot];
-  float a = 0, b = 0;
+  float a = 0, b = 0, m = 0;
   if (hipEventElapsedTime(&a, s.ev0, s.ev1) != hipSuccess) a = 0;
   if (hipEventElapsedTime(&b, s.ev1, s.ev2) != hipSuccess) b = 0;
+  if (s.n_bytes == 0 || hipEventElapsedTime(&m, s.evm0, s.evm1) != hipSuccess) m = 0;
   if (h2d_ms) *h2d_ms = a;
   if (kernel_ms) *kernel_ms = b;
+  if (main_ms) *main_ms = m;
   return WARPJQ_OK;
 }
 
 uint32_t warpjq_amd_kernel_width(const warpjq_ctx *ctx) { return ctx ? ctx->host.ns : 0; }
+
+warpjq_status warpjq_amd_set_claim(warpjq_ctx *ctx, uint32_t claim_bytes) {
+  if (!ctx || claim_bytes < CLAIM_MIN || claim_bytes > CLAIM_MAX || claim_bytes % STEP_BYTES) return WARPJQ_ERR_INVALID_ARG;
+  ctx->claim = claim_bytes;
+  return WARPJQ_OK;
+}
 
 }  // extern "C"
