@@ -51,6 +51,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="only the timed filter-count loop")
     ap.add_argument("--cpu-sample-mib", type=int, default=1024)
     ap.add_argument("--claim-kib", type=int, default=0, help="bytes per wave claim (tuning)")
+    ap.add_argument("--arena-kib", type=int, default=0, help="LDS arena per wave (tuning)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -74,6 +75,8 @@ def main():
     base = d.data_ptr()
     pieces = split_points(host, n, args.piece_mib << 20)
     ctx = w.Context(QUERY, chunk_bytes=64 << 20, n_slots=1)
+    if args.arena_kib:
+        ctx.set_arena(args.arena_kib << 10)
     if args.claim_kib:
         ctx.set_claim(args.claim_kib << 10)
 
@@ -174,7 +177,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "nginx NDJSON, select(.status == 500) | count, resident in HBM",
                    "bytes": n, "lines": n_lines, "avg_line_bytes": round(n / n_lines, 1),
-                   "pieces": len(pieces), "device": dev},
+                   "pieces": len(pieces), "device": dev, "grid": ctx.grid},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
                      "kernel": "k_main", "main_ms_per_step": round(main_ms, 3)},

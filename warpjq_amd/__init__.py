@@ -90,7 +90,8 @@ REFERENCE_SYMBOLS = ["warpjq_probe", "warpjq_abi_check", "warpjq_ctx_create", "w
                      "warpjq_slot_buffer", "warpjq_slot_capacity", "warpjq_max_lines", "warpjq_submit",
                      "warpjq_wait", "warpjq_device_name"]
 ADDITIVE_SYMBOLS = ["warpjq_amd_ext_value", "warpjq_amd_wait_ext", "warpjq_amd_submit_device",
-                    "warpjq_amd_slot_timing", "warpjq_amd_kernel_width", "warpjq_amd_set_claim", "warpjq_amd_run_buffer",
+                    "warpjq_amd_slot_timing", "warpjq_amd_kernel_width", "warpjq_amd_set_claim", "warpjq_amd_set_arena",
+                    "warpjq_amd_grid", "warpjq_amd_run_buffer",
                     "warpjq_amd_free", "warpjq_amd_compile", "warpjq_amd_program_view",
                     "warpjq_amd_program_free", "warpjq_amd_gen_nginx"]
 
@@ -133,6 +134,9 @@ def lib():
         L.warpjq_amd_kernel_width.argtypes = [ctypes.c_void_p]
         L.warpjq_amd_kernel_width.restype = u32
         L.warpjq_amd_set_claim.argtypes = [ctypes.c_void_p, u32]
+        L.warpjq_amd_set_arena.argtypes = [ctypes.c_void_p, u32]
+        L.warpjq_amd_grid.argtypes = [ctypes.c_void_p]
+        L.warpjq_amd_grid.restype = u32
         L.warpjq_amd_run_buffer.argtypes = [cp, ctypes.c_void_p, u64, ctypes.c_int, ctypes.c_int, u64, u64,
                                             P(P(ctypes.c_uint8)), P(u64), P(Stats), cp, sz]
         L.warpjq_amd_run_buffer.restype = ctypes.c_int
@@ -258,7 +262,15 @@ class Context:
 
     def set_claim(self, nbytes):
         if lib().warpjq_amd_set_claim(self.ctx, nbytes) != 0:
-            raise ValueError("claim must be 1-64 KiB in 1 KiB steps")
+            raise ValueError("claim must be 1-16 KiB in 1 KiB steps and fit the arena with 1 KiB to spare")
+
+    def set_arena(self, nbytes):
+        if lib().warpjq_amd_set_arena(self.ctx, nbytes) != 0:
+            raise ValueError("arena must be a multiple of 256 B, at most 32 KiB, at least the claim plus 1 KiB")
+
+    @property
+    def grid(self):
+        return lib().warpjq_amd_grid(self.ctx)
 
     @property
     def capacity(self):

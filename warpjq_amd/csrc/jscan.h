@@ -1187,14 +1187,17 @@ WJQ_HD uint32_t group_hash(R &rd, const GKey &k) {
   return h | 1u;  // 0 is reserved for "not yet written"
 }
 
-// Equality of two group keys from the same chunk.
-template <class R>
-WJQ_HD bool group_eq(R &rd, const GKey &a, const GKey &b) {
+// Equality of two group keys from the same chunk; `ra` reads key `a` and `rb`
+// reads key `b` (the kernels read the current line from LDS and a stored key
+// from HBM).
+template <class RA, class RB>
+WJQ_HD bool group_eq(RA &ra, const GKey &a, RB &rb, const GKey &b) {
   if (a.kind != b.kind) return false;
   if (a.kind == JK_NULL) return true;
   if (a.kind == JK_BOOL) return a.len == b.len;
   if (a.kind == JK_STR && (a.esc || b.esc)) {
-    DecIter<R> x(rd, a.off, a.off + a.len), y(rd, b.off, b.off + b.len);
+    DecIter<RA> x(ra, a.off, a.off + a.len);
+    DecIter<RB> y(rb, b.off, b.off + b.len);
     for (;;) {
       int c = x.next(), e = y.next();
       if (c != e) return false;
@@ -1203,7 +1206,7 @@ WJQ_HD bool group_eq(R &rd, const GKey &a, const GKey &b) {
   }
   if (a.len != b.len) return false;
   for (uint32_t q = 0; q < a.len; q++)
-    if (rd.byte(a.off + q) != rd.byte(b.off + q)) return false;
+    if (ra.byte(a.off + q) != rb.byte(b.off + q)) return false;
   return true;
 }
 

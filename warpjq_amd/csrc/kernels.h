@@ -5,16 +5,24 @@
 //   H2D (pinned staging -> HBM)
 //   k_reset        counters, tile states, ticket
 //   k_main<NS>     ONE pass over the chunk. Persistent single-wave workgroups
-//                  claim 16 KiB tiles in order from a ticket. Each tile is
-//                  read once with coalesced 16-byte loads (64 lanes x 16 B =
-//                  1 KiB per step); per-lane newline masks and a wave prefix
-//                  sum turn it into line-start offsets queued in LDS. The
-//                  tile publishes its line count and takes its first line's
-//                  global index by decoupled look-back. Then every lane
-//                  evaluates one queued line at a time (jscan.h: validate +
-//                  capture + filter), so lines from consecutive tiles fill all
-//                  64 lanes. Aggregates stay in registers; group_by goes
-//                  through a per-wave LDS table into an HBM hash table.
+//                  claim tiles ("claims", a few KiB) in order from a ticket.
+//                  A claim is copied HBM -> LDS with direct-to-LDS 16-byte
+//                  loads (global_load_lds_dwordx4) into a per-wave LDS
+//                  "arena", followed by the first bytes of the next claim up
+//                  to the end of the claim's last line. Newline masks and a
+//                  wave prefix sum over the arena give the claim's line
+//                  starts, queued in LDS. Every lane then evaluates one queued
+//                  line at a time (jscan.h: validate + capture + filter),
+//                  reading only LDS, so a round of 64 lines mixes lines of
+//                  consecutive claims and every lane stays busy; the arena is
+//                  compacted before each new claim. Aggregates stay in
+//                  registers; group_by goes through a per-wave LDS table into
+//                  an HBM hash table. Row outputs need each line's index in
+//                  the chunk: the claim takes its first index by decoupled
+//                  look-back. Aggregate queries skip the look-back and store
+//                  per-claim line counts; the rare records that need an index
+//                  (first invalid line, declined lines) carry (claim, rank)
+//                  and are resolved after the pass.
 //   k_agg_final    ungrouped partials -> the chunk's warpjq_agg_partial
 //   k_group_compact occupied HBM group slots -> warpjq_group[], clearing them
 //   k_row_*        (row outputs) scan of per-line row lengths -> ascending
@@ -22,6 +30,11 @@
 //   k_emit<NS>     re-evaluates each selected line and writes its row
 //   D2H of the fixed-size results (counters, aggregate)
 // warpjq_wait synchronises the stream and copies the variable-size results.
+//
+// Positions inside the kernels are "q positions": offsets from the chunk's
+// 16-byte-aligned-down base, so the chunk's first byte is at q = mis (its
+// misalignment, 0..15). All 16-byte loads are then aligned, and arena offsets
+// agree with q positions modulo 16.
 //
 // Reference ABI semantics followed: kernels.cu:2233-2685 (probe, abi_check,
 // ctx_create sizing, submit, wait), kernels.h:155-230 (result meaning).
@@ -43,16 +56,23 @@
 
 using namespace wjqd;
 
+#define LDS_AS __attribute__((address_space(3)))
+
 // ---------------------------------------------------------------------------
 // tunables
 // ---------------------------------------------------------------------------
 
 #define CLAIM_MIN 1024u              // smallest claim a wave takes (one scan step)
-#define CLAIM_MAX 65536u             // largest claim (the queue index is 16-bit safe)
+#define CLAIM_MAX 16384u             // largest claim
+#define CLAIM_DEFAULT 4096u
 #define STEP_BYTES 1024u             // 64 lanes x 16 B
-#define STAGE_BYTES 16384u           // LDS staging of one round's line span
-#define QCAP 512u                    // per-wave LDS queue of line starts
+#define TAIL_PIECE 256u              // bytes of the next claim fetched per try
+#define TAIL_MAX 1024u               // a claim's last line may end this far past it
+#define ARENA_DEFAULT 20480u         // per-wave LDS arena (bytes)
+#define ARENA_MAX 32768u             // offsets fit 15 bits; bit 15 marks unstaged lines
+#define QCAP 256u                    // per-wave LDS queue of line starts
 #define QMASK (QCAP - 1u)
+#define Q_UNSTAGED 0x8000u
 #define LGROUP 64u                   // per-wave LDS group table entries
 #define GROUP_MAX_PROBE 128u
 #define MIN_LINE_BYTES 24u           // kernels.cu:1984-1988 sizing rule
@@ -67,6 +87,7 @@ using namespace wjqd;
 
 struct DevCtr {
   unsigned long long n_lines, n_blank, n_invalid, n_type_error, n_fallback, n_sel, out_total;
+  unsigned long long first_bad;  // (claim << 32 | rank) of the first invalid line, aggregate queries
   unsigned int first_invalid, overflow, saw_non_numeric, group_overflow;
   unsigned int fb_count, n_groups, ticket, tiles_done;
 };
@@ -88,47 +109,45 @@ struct GroupTab {
 };
 
 struct KArgs {
-  const uint8_t *d;
-  uint32_t n;
+  const uint8_t *d;  // 16-byte-aligned base; the chunk starts at d + mis
+  uint32_t mis;
+  uint32_t nq;       // mis + chunk bytes
   uint32_t ntiles;
-  uint32_t claim;  // bytes per claimed tile (multiple of STEP_BYTES)
+  uint32_t claim;    // bytes per claim (multiple of STEP_BYTES)
+  uint32_t arena;    // LDS arena bytes per wave (multiple of 256)
   DProg P;
   DevCtr *ctr;
+  // row outputs: look-back states; aggregates: line count per claim
   unsigned long long *tile_state;
   AggPart *parts;
   GroupTab gt;
   // row outputs
   uint32_t *row_len;    // per line (0 = not selected)
-  uint32_t *sel_start;  // per line, written for selected lines
+  uint32_t *sel_start;  // per line, q position, written for selected lines
   uint64_t max_lines;
-  // fallback list
-  uint32_t *fb_idx, *fb_off, *fb_len;
+  // declined lines (ST_FALLBACK)
+  uint32_t *fb_idx, *fb_off, *fb_len, *fb_tile;
   uint32_t fb_cap;
 };
 
 // ---------------------------------------------------------------------------
-// reading the chunk
+// readers
 // ---------------------------------------------------------------------------
 
-// Caches one 16-byte block per lane; positions at or past n read as '\n'.
-// Blocks inside the round's staged span [slo, shi) come from LDS, the rest
-// from global memory (L2).
+// Reads the chunk from HBM; caches one 16-byte block. Positions at or past
+// n read as '\n'.
 struct DevReader {
   const uint8_t *d;
   uint32_t n;
   uint32_t cb;
   uint32_t v0, v1, v2, v3;
-  const uint4 *stg;
-  uint32_t slo, shi;
 
-  __device__ DevReader(const uint8_t *d_, uint32_t n_)
-      : d(d_), n(n_), cb(0xFFFFFFFFu), v0(0), v1(0), v2(0), v3(0), stg(nullptr), slo(0), shi(0) {}
+  __device__ DevReader(const uint8_t *d_, uint32_t n_) : d(d_), n(n_), cb(0xFFFFFFFFu), v0(0), v1(0), v2(0), v3(0) {}
 
   __device__ __forceinline__ uint32_t dword(uint32_t k) {
     uint32_t b = k >> 2;
     if (b != cb) {
-      uint32_t off = (b << 4) - slo;
-      uint4 v = off < shi - slo ? stg[off >> 4] : *((const uint4 *)d + b);
+      uint4 v = *((const uint4 *)d + b);
       v0 = v.x;
       v1 = v.y;
       v2 = v.z;
@@ -151,12 +170,37 @@ struct DevReader {
   }
 };
 
+// Reads a line staged in the wave's LDS arena: position i lives at arena
+// byte i + delta. Callers only read up to the line's terminator, which is
+// staged (bytes past the chunk end are staged as '\n').
+struct LdsReader {
+  const LDS_AS uint8_t *b;
+  uint32_t delta;
+  __device__ __forceinline__ uint32_t byte(uint32_t i) const { return b[i + delta]; }
+  __device__ __forceinline__ uint32_t dword(uint32_t k) const {
+    return *(const LDS_AS uint32_t *)(b + ((k << 2) + delta));
+  }
+};
+
+// 16-byte LDS accesses through address-space-3 pointers (ds_read_b128 /
+// ds_write_b128).
+typedef unsigned int wjq_v4u __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 lds_ld16(const LDS_AS uint8_t *p) {
+  wjq_v4u v = *(const LDS_AS wjq_v4u *)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void lds_st16(LDS_AS uint8_t *p, uint4 v) {
+  wjq_v4u x = {v.x, v.y, v.z, v.w};
+  *(LDS_AS wjq_v4u *)p = x;
+}
+
 // Byte sink writing a row into HBM, bounded by the output capacity.
 struct DevSink {
   uint8_t *out;
   uint64_t pos;
   __device__ __forceinline__ void put(uint8_t c) { out[pos++] = c; }
-  __device__ __forceinline__ void span(DevReader &rd, uint32_t off, uint32_t len) {
+  template <class R>
+  __device__ __forceinline__ void span(R &rd, uint32_t off, uint32_t len) {
     for (uint32_t k = 0; k < len; k++) out[pos + k] = (uint8_t)rd.byte(off + k);
     pos += len;
   }
@@ -203,13 +247,39 @@ __device__ __forceinline__ void st_rel(unsigned long long *p, unsigned long long
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Bit k set when byte k of the 16 is '\n' (exact per byte).
+__device__ __forceinline__ uint32_t nl_mask16(uint4 v) {
+  uint32_t w[4] = {v.x, v.y, v.z, v.w};
+  uint32_t m = 0;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    uint32_t x = w[q] ^ 0x0A0A0A0Au;
+    uint32_t z = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
+    uint32_t bits = ((z >> 7) & 1) | ((z >> 14) & 2) | ((z >> 21) & 4) | ((z >> 28) & 8);
+    m |= bits << (4 * q);
+  }
+  return m;
+}
+
+// The 16 bytes at q position p with every byte at or past n replaced by '\n'.
+__device__ __forceinline__ uint4 end_patch(uint4 v, uint32_t p, uint32_t n) {
+  uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (uint32_t q = 0; q < 4; q++) {
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++)
+      if (p + 4 * q + k >= n) w[q] = (w[q] & ~(0xFFu << (8 * k))) | (WJQ_END << (8 * k));
+  }
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 #define FLAG_A (1ull << 62)
 #define FLAG_P (2ull << 62)
 #define VAL_MASK ((1ull << 62) - 1)
 
 // Decoupled look-back: publishes this tile's count and returns the sum of
 // counts of all earlier tiles. Wave-uniform.
-__device__ uint64_t tile_lookback(unsigned long long *state, uint32_t t, uint64_t agg) {
+__device__ __forceinline__ uint64_t tile_lookback(unsigned long long *state, uint32_t t, uint64_t agg) {
   uint32_t lane = lane_id();
   if (t == 0) {
     if (lane == 0) st_rel(&state[0], FLAG_P | agg);
@@ -293,9 +363,11 @@ __device__ __forceinline__ void group_apply(const GroupTab &g, uint32_t pos, con
   }
 }
 
-// Finds or claims the slot of `k` in the HBM table; ~0u on overflow. The key
-// is published whole by one CAS, so a reader never sees a half-written key.
-__device__ uint32_t group_find(DevReader &rd, const GroupTab &g, const GKey &k, uint32_t h) {
+// Finds or claims the slot of `k` (read through `rk`) in the HBM table; ~0u
+// on overflow. Stored keys are read from HBM through `gr`. The key is
+// published whole by one CAS, so a reader never sees a half-written key.
+template <class RK>
+__device__ __forceinline__ uint32_t group_find(RK &rk, DevReader &gr, const GroupTab &g, const GKey &k, uint32_t h) {
   unsigned long long pk = pack_key(k);
   uint32_t pos = h & (g.cap - 1);
   for (uint32_t probe = 0; probe < GROUP_MAX_PROBE; probe++) {
@@ -310,7 +382,7 @@ __device__ uint32_t group_find(DevReader &rd, const GroupTab &g, const GKey &k, 
     }
     unsigned int hv = __hip_atomic_load(&g.hash[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (hv == 0 || hv == h) {
-      if (cur == pk || group_eq(rd, k, unpack_key(cur))) return pos;
+      if (cur == pk || group_eq(rk, k, gr, unpack_key(cur))) return pos;
     }
     pos = (pos + 1) & (g.cap - 1);
   }
@@ -334,8 +406,9 @@ __device__ __forceinline__ void lds_acc_f64(double *s, double *c, double v) {
 }
 
 // Inserts one line's value; falls through to HBM when the LDS table is full.
-__device__ void group_line(DevReader &rd, const GroupTab &g, LGroup &L, const GKey &k, uint32_t h, bool has_num,
-                           bool is_int, int64_t iv, double v, unsigned int *ovf) {
+template <class R>
+__device__ __forceinline__ void group_line(R &rd, DevReader &gr, const GroupTab &g, LGroup &L, const GKey &k, uint32_t h,
+                           bool has_num, bool is_int, int64_t iv, double v, unsigned int &ovf) {
   unsigned long long pk = pack_key(k);
   uint32_t pos = h & (LGROUP - 1);
   int found = -1;
@@ -351,7 +424,7 @@ __device__ void group_line(DevReader &rd, const GroupTab &g, LGroup &L, const GK
       cur = old;
     }
     unsigned int hv = L.hash[pos];
-    if ((hv == 0 || hv == h) && (cur == pk || group_eq(rd, k, unpack_key(cur)))) {
+    if ((hv == 0 || hv == h) && (cur == pk || group_eq(rd, k, gr, unpack_key(cur)))) {
       found = (int)pos;
       break;
     }
@@ -385,9 +458,9 @@ __device__ void group_line(DevReader &rd, const GroupTab &g, LGroup &L, const GK
     }
     return;
   }
-  uint32_t gp = group_find(rd, g, k, h);
+  uint32_t gp = group_find(rd, gr, g, k, h);
   if (gp == ~0u) {
-    atomicOr(ovf, 1u);
+    ovf = 1;
     return;
   }
   GVal gv;
@@ -400,6 +473,124 @@ __device__ void group_line(DevReader &rd, const GroupTab &g, LGroup &L, const GK
   gv.mnb = has_num ? ordered_bits(v) : ~0ull;
   gv.mxb = has_num ? ordered_bits(v) : 0ull;
   group_apply(g, gp, gv);
+}
+
+// ---------------------------------------------------------------------------
+// k_main: per-line work
+// ---------------------------------------------------------------------------
+
+// Per-lane tallies, flushed once at the end of the kernel.
+struct LaneAcc {
+  uint32_t blank, inv, te, fb, sel;
+  uint64_t cnt, num, ilo, ihi;
+  double fs, fc, mn, mx;
+  bool saw_nn;
+  unsigned int ovf, govf;
+};
+
+// One line: evaluate, then fold its outcome into the lane's tallies, the
+// group table or the per-line row arrays. `rec` is the line's index in the
+// chunk (row outputs) or its rank in claim `tile` (aggregates).
+template <int NS, bool GRP, bool LB, class R>
+__device__ __forceinline__ void process_line(R &rd, const KArgs &a, uint32_t s, uint32_t rec, uint32_t tile,
+                                             LaneAcc &L, LGroup *lgp) {
+  const DProg &P = a.P;
+  SlotV sl[NS];
+  LineResult r = eval_line<NS>(rd, s, P, sl);
+  if (r.status == ST_OK && r.selected && P.output_kind == WARPJQ_OUT_AGG) {
+    bool has_num = false, is_int = false, undecided = false;
+    double v = 0.0;
+    int64_t iv = 0;
+    if (P.agg_slot != 0xFFFFFFFFu) {
+      SlotV x = pick<NS>(sl, P.agg_slot);
+      if (x.kind == JK_NUM) {
+        NumVal nv;
+        if (parse_number(rd, x.off, x.off + x.len, nv)) {
+          has_num = true;
+          is_int = nv.is_int;
+          iv = nv.iv;
+          v = nv.v;
+        } else {
+          undecided = true;
+        }
+      } else if (x.kind != JK_MISSING && x.kind != JK_NULL) {
+        L.saw_nn = true;
+      }
+    }
+    GKey gk{0, 0, 0, 0};
+    uint32_t gh = 0;
+    if (!undecided && GRP) {
+      gk = group_key(rd, pick<NS>(sl, P.group_slot));
+      if (gk.len > 0x7FFFF) undecided = true;
+      else gh = group_hash(rd, gk);
+    }
+    if (undecided) {
+      r.status = ST_FALLBACK;
+    } else if (GRP) {
+      DevReader gr(a.d, a.nq);
+      group_line(rd, gr, a.gt, *lgp, gk, gh, has_num, is_int, iv, v, L.govf);
+    } else {
+      L.cnt++;
+      if (has_num) {
+        L.num++;
+        if (is_int) {
+          uint64_t lo = (uint64_t)iv;
+          uint64_t old = L.ilo;
+          L.ilo += lo;
+          L.ihi += (iv < 0 ? ~0ull : 0ull) + (L.ilo < old ? 1 : 0);
+        } else {
+          double t = L.fs + v;  // Neumaier
+          if (fabs(L.fs) >= fabs(v)) L.fc += (L.fs - t) + v;
+          else L.fc += (v - t) + L.fs;
+          L.fs = t;
+        }
+        if (v < L.mn) L.mn = v;
+        if (v > L.mx) L.mx = v;
+      }
+    }
+  }
+  if (LB) {
+    uint32_t rl = 0;
+    if (r.status == ST_OK && r.selected) {
+      uint64_t n = row<NS, R, DevSink>(rd, P, sl, s, r.end, nullptr);
+      rl = (uint32_t)n;
+      L.sel++;
+    }
+    if (rec < a.max_lines) {
+      a.row_len[rec] = rl;
+      if (rl) a.sel_start[rec] = s;
+    } else {
+      L.ovf = 1;
+    }
+  }
+  switch (r.status) {
+    case ST_BLANK: L.blank++; break;
+    case ST_INVALID:
+      L.inv++;
+      if (LB) atomicMin(&a.ctr->first_invalid, rec);
+      else atomicMin(&a.ctr->first_bad, ((unsigned long long)tile << 32) | rec);
+      break;
+    case ST_TYPE_ERROR: L.te++; break;
+    case ST_FALLBACK: {
+      L.fb++;
+      uint32_t e = s;
+      while (rd.byte(e) != WJQ_END) e++;
+      uint32_t len = e - s;
+      if (len > 0 && rd.byte(e - 1) == '\r') len--;
+      uint32_t k = atomicAdd(&a.ctr->fb_count, 1u);
+      if (k < a.fb_cap) {
+        a.fb_idx[k] = rec;
+        a.fb_tile[k] = tile;
+        a.fb_off[k] = s - a.mis;
+        a.fb_len[k] = len;
+      } else {
+        L.ovf = 1;
+      }
+      if (LB && rec < a.max_lines) a.row_len[rec] = 0;
+      break;
+    }
+    default: break;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -418,17 +609,121 @@ struct LGroupStore<false> {
   __device__ static LGroup *get() { return nullptr; }
 };
 
-template <int NS, bool GRP>
+extern __shared__ uint4 wjq_dyn_lds[];
+
+// Dynamic LDS: [arena | qs | qi | qt | qo].
+__host__ __device__ constexpr uint32_t main_lds_bytes(uint32_t arena) { return arena + QCAP * (4 + 4 + 4 + 2); }
+
+// The queue of line starts (an LDS ring of QCAP entries): q position, index
+// or rank, claim, arena offset (bit 15: the line is not wholly staged).
+struct LQueue {
+  LDS_AS uint32_t *qs, *qi, *qt;
+  LDS_AS uint16_t *qo;
+};
+
+// The claim whose line starts are still being queued.
+struct Claim {
+  uint32_t t, t0, end;  // claim index, first q position, end of its line starts
+  uint32_t at;          // arena offset of q position t0
+  uint32_t resume;      // first line start not yet queued (q position)
+  uint32_t rank;        // its rank in the claim
+  uint32_t base;        // index of the claim's first line (row outputs)
+  uint32_t carry0;      // 1 when a line starts at t0
+  uint32_t last_unstaged;  // start of the claim's last line when it is not wholly staged, else ~0
+};
+
+// Queues line starts of claim `c` from c.resume on, at most `room` of them;
+// when `count` is set it scans the whole claim and returns its line count.
+// Advances c.resume / c.rank; returns the number queued in *queued.
+// Wave-uniform.
+__device__ __forceinline__ uint32_t enqueue_lines(const KArgs &a, const LDS_AS uint8_t *ar, LQueue &Q, uint32_t qtail,
+                                  uint32_t room, Claim &c, bool count, uint32_t *queued) {
+  const uint32_t lane = lane_id();
+  uint32_t k0 = (c.resume - c.t0) / STEP_BYTES;
+  uint32_t nsteps = a.claim / STEP_BYTES;
+  uint32_t carry;
+  if (k0 == 0) {
+    carry = c.carry0;
+  } else {
+    uint32_t pb = c.t0 + k0 * STEP_BYTES - 1;
+    carry = ar[pb - c.t0 + c.at] == '\n' ? 1u : 0u;
+  }
+  uint32_t seen = 0;       // starts >= c.resume seen in this pass
+  uint32_t next = c.end;   // first unqueued start
+  bool cut = false;
+  for (uint32_t k = k0; k < nsteps; k++) {
+    const uint32_t sb = c.t0 + k * STEP_BYTES;
+    if (sb >= c.end || (cut && !count)) break;
+    const uint32_t p = sb + lane * 16u;
+    uint4 v = lds_ld16(ar + (p - c.t0 + c.at));
+    uint32_t m = nl_mask16(v);
+    uint32_t up = __shfl_up(m >> 15, 1, 64) & 1u;
+    uint32_t prev = lane == 0 ? carry : up;
+    carry = __shfl(m >> 15, 63, 64) & 1u;
+    uint32_t st = ((m << 1) | prev) & 0xFFFFu;
+    // the chunk's first line starts at q = mis; nothing before it does
+    if (p + 16u <= a.mis) st = 0;
+    else if (p <= a.mis) st = (st & (0xFFFFu << (a.mis - p))) | (1u << (a.mis - p));
+    // keep starts in [c.resume, c.end)
+    if (p + 16u <= c.resume) st = 0;
+    else if (p < c.resume) st &= 0xFFFFu << (c.resume - p);
+    if (p >= c.end) st = 0;
+    else if (p + 16u > c.end) st &= (1u << (c.end - p)) - 1u;
+    uint32_t cnt = __popc(st);
+    uint32_t tot;
+    uint32_t ex = wave_excl_scan(cnt, tot);
+    if (!cut) {
+      uint32_t r = seen + ex;
+      uint32_t bits = st;
+      while (bits) {
+        uint32_t b = __builtin_ctz(bits);
+        bits &= bits - 1;
+        if (r < room) {
+          uint32_t q = p + b;
+          uint32_t e = (qtail + r) & QMASK;
+          Q.qs[e] = q;
+          Q.qi[e] = c.rank + r;
+          Q.qt[e] = c.t;
+          Q.qo[e] = (uint16_t)((q - c.t0 + c.at) | (q == c.last_unstaged ? Q_UNSTAGED : 0u));
+        } else if (r == room) {
+          next = p + b;  // exactly one lane sees rank `room`
+        }
+        r++;
+      }
+      if (seen + tot > room) {
+        cut = true;
+        // broadcast the position of the start ranked `room`
+        uint64_t own = __ballot(next != c.end);
+        uint32_t src = (uint32_t)__builtin_ctzll(own);
+        next = __shfl(next, (int)src, 64);
+      }
+    }
+    seen += tot;
+  }
+  uint32_t q = seen < room ? seen : room;
+  *queued = q;
+  if (cut) {
+    c.resume = next;
+    c.rank += room;
+  } else {
+    c.resume = c.end;
+    c.rank += seen;
+  }
+  return seen;
+}
+
+template <int NS, bool GRP, bool LB>
 __global__ __launch_bounds__(64) void k_main(KArgs a) {
-  __shared__ uint32_t qs[QCAP];
-  __shared__ uint32_t qi[QCAP];
-  __shared__ uint4 stage[STAGE_BYTES / 16];
+  LDS_AS uint8_t *ar = (LDS_AS uint8_t *)wjq_dyn_lds;
+  LQueue Q;
+  Q.qs = (LDS_AS uint32_t *)(ar + a.arena);
+  Q.qi = Q.qs + QCAP;
+  Q.qt = Q.qi + QCAP;
+  Q.qo = (LDS_AS uint16_t *)(Q.qt + QCAP);
   LGroup *lgp = LGroupStore<GRP>::get();
 
   const uint32_t lane = lane_id();
   const DProg &P = a.P;
-  const bool grouped = GRP;
-  const bool rows = P.output_kind != WARPJQ_OUT_AGG;
 
   if (GRP) {
     LGroup &lg = *lgp;
@@ -443,241 +738,171 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
     __syncthreads();
   }
 
-  // per-lane accumulators
-  uint32_t c_blank = 0, c_inv = 0, c_te = 0, c_fb = 0, c_sel = 0;
-  uint64_t lines_total = 0;  // wave-uniform
-  uint64_t a_cnt = 0, a_num = 0, a_ilo = 0, a_ihi = 0;
-  double a_fs = 0.0, a_fc = 0.0, a_mn = INFINITY, a_mx = -INFINITY;
-  bool saw_nn = false;
-  unsigned int ovf_local = 0;   // chunk-level capacity exceeded
-  unsigned int govf_local = 0;  // group table overflow
+  LaneAcc L;
+  L.blank = L.inv = L.te = L.fb = L.sel = 0;
+  L.cnt = L.num = L.ilo = L.ihi = 0;
+  L.fs = L.fc = 0.0;
+  L.mn = INFINITY;
+  L.mx = -INFINITY;
+  L.saw_nn = false;
+  L.ovf = L.govf = 0;
 
+  uint64_t lines_total = 0;     // wave-uniform
   uint32_t qhead = 0, qlen = 0;  // wave-uniform
-  // a tile whose line starts did not all fit in the queue
-  bool pend = false;
-  uint32_t pend_t = 0, pend_step = 0;
-  uint64_t pend_idx = 0;
+  uint32_t used = 0;             // arena bytes holding queued lines (from 0)
   bool tiles_left = true;
-
-  DevReader rd(a.d, a.n);
-  rd.stg = stage;
-  uint32_t claim_end = 0;
+  bool pending = false;          // a claim still has unqueued line starts
+  Claim c{};
+  const uint32_t need = a.claim + TAIL_MAX;
 
   for (;;) {
-    // ---- refill: rounds never mix claims, so a round's lines are one
-    // contiguous span of the chunk ----
-    while (qlen == 0 && (pend || tiles_left)) {
-      qhead = 0;
-      uint32_t t, step0;
-      uint64_t idx0 = 0;
-      bool known_base;
-      if (pend) {
-        t = pend_t;
-        step0 = pend_step;
-        idx0 = pend_idx;
-        known_base = true;
-        pend = false;
+    // ---- fill the queue to one full round when possible ----
+    while (qlen < 64) {
+      if (pending) {
+        uint32_t got;
+        enqueue_lines(a, ar, Q, qhead + qlen, QCAP - qlen, c, false, &got);
+        if (LB)
+          for (uint32_t e = lane; e < got; e += 64) Q.qi[(qhead + qlen + e) & QMASK] += c.base;
+        qlen += got;
+        pending = c.resume < c.end;
+        __syncthreads();
+        if (pending) break;  // the queue is full
+        continue;
+      }
+      if (!tiles_left) break;
+      // compact: keep the bytes of queued lines at the front of the arena
+      if (qlen == 0) {
+        used = 0;
       } else {
-        uint32_t tk = 0;
-        if (lane == 0) tk = atomicAdd(&a.ctr->ticket, 1u);
-        t = __shfl(tk, 0, 64);
-        if (t >= a.ntiles) {
-          tiles_left = false;
-          break;
+        uint32_t lo = (uint32_t)(Q.qo[qhead & QMASK] & 0x7FFFu) & ~15u;
+        if (lo) {
+          for (uint32_t o = 0; o < used - lo; o += STEP_BYTES) {
+            uint32_t x = o + lane * 16u;
+            uint4 v;
+            if (x < used - lo) v = lds_ld16(ar + lo + x);
+            __syncthreads();
+            if (x < used - lo) lds_st16(ar + x, v);
+            __syncthreads();
+          }
+          for (uint32_t e = lane; e < qlen; e += 64) {
+            uint32_t k = (qhead + e) & QMASK;
+            Q.qo[k] = (uint16_t)(Q.qo[k] - lo);
+          }
+          used -= lo;
+          __syncthreads();
         }
-        step0 = 0;
-        known_base = false;
       }
+      if (a.arena - used < need) break;  // run a (partial) round first
+      uint32_t tk = 0;
+      if (lane == 0) tk = atomicAdd(&a.ctr->ticket, 1u);
+      const uint32_t t = __shfl(tk, 0, 64);
+      if (t >= a.ntiles) {
+        tiles_left = false;
+        break;
+      }
+      // ---- fetch claim t and its first tail piece into the arena ----
       const uint32_t t0 = t * a.claim;
-      const uint32_t tq0 = qhead + qlen;  // first queue entry of this tile
-      uint32_t local = 0;                 // line starts seen in this tile from step0
-      uint32_t stored = 0;
-      bool full = false;
-      uint32_t carry;
-      {
-        uint32_t p0 = t0 + step0 * STEP_BYTES;
-        uint32_t c = 1;
-        if (p0 > 0) c = a.d[p0 - 1] == '\n' ? 1 : 0;
-        carry = c;
+      const uint32_t at = used;
+      for (uint32_t j = 0; j < a.claim + TAIL_PIECE; j += STEP_BYTES) {
+        uint32_t x = j + lane * 16u;
+        if (x < a.claim + TAIL_PIECE && t0 + x < a.nq)
+          __builtin_amdgcn_global_load_lds((const void *)(a.d + t0 + x), (LDS_AS void *)(ar + at + j), 16, 0, 0);
       }
-      for (uint32_t step = step0; step < a.claim / STEP_BYTES; step++) {
-        uint32_t p = t0 + step * STEP_BYTES + lane * 16;
-        if (t0 + step * STEP_BYTES >= a.n) break;
-        uint32_t m = 0;
-        if (p < a.n) {
-          uint4 v = *((const uint4 *)(a.d + p));
-          uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            uint32_t x = w[q] ^ 0x0A0A0A0Au;
-            uint32_t z = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);  // exact zero bytes
-            uint32_t bits = ((z >> 7) & 1) | ((z >> 14) & 2) | ((z >> 21) & 4) | ((z >> 28) & 8);
-            m |= bits << (4 * q);
-          }
-          if (p + 16 > a.n) m &= (1u << (a.n - p)) - 1u;
-        }
-        uint32_t up = __shfl_up(m >> 15, 1, 64) & 1u;
-        uint32_t prev = lane == 0 ? carry : up;
-        uint32_t st = ((m << 1) | prev) & 0xFFFFu;
-        if (p >= a.n) st = 0;
-        else if (p + 16 > a.n) st &= (1u << (a.n - p)) - 1u;
-        carry = __shfl(m >> 15, 63, 64) & 1u;
-        uint32_t cnt = __popc(st);
-        uint32_t tot;
-        uint32_t ex = wave_excl_scan(cnt, tot);
-        if (!full && qlen + stored + tot > QCAP) {
-          full = true;
-          pend_step = step;
-          pend_idx = local;  // local rank of the first unstored line
-        }
-        if (!full) {
-          uint32_t r = 0;
-          while (st) {
-            uint32_t b = __builtin_ctz(st);
-            st &= st - 1;
-            uint32_t e = (tq0 + stored + ex + r) & QMASK;
-            qs[e] = p + b;
-            qi[e] = (uint32_t)(idx0 + local + ex + r);  // local rank, or global when the base is known
-            r++;
-          }
-          stored += tot;
-        }
-        local += tot;
-      }
+      uint32_t carry0 = 1;
+      if (t0 > 0) carry0 = (*(const uint32_t *)(a.d + t0 - 4) >> 24) == '\n' ? 1u : 0u;
+      __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
-      if (!known_base) {
-        uint64_t base = tile_lookback(a.tile_state, t, local);
-        lines_total += local;
-        for (uint32_t e = lane; e < stored; e += 64) qi[(tq0 + e) & QMASK] += (uint32_t)base;
-        if (full) {
-          pend = true;
-          pend_t = t;
-          pend_idx += base;
+      uint32_t span = a.claim + TAIL_PIECE;  // bytes staged for this claim
+      if (t0 + span > a.nq) {
+        for (uint32_t x = lane * 16u; x < span; x += STEP_BYTES) {
+          uint32_t p = t0 + x;
+          if (p + 16u > a.nq) lds_st16(ar + at + x, end_patch(lds_ld16(ar + at + x), p, a.nq));
         }
-      } else if (full) {
-        pend = true;
-        pend_t = t;
-        pend_idx += idx0;
+        __syncthreads();
       }
-      qlen += stored;
-      claim_end = t0 + a.claim;
+      // the claim's last line must end inside the staged bytes
+      bool found = t0 + a.claim >= a.nq || ar[at + a.claim - 1] == '\n';
+      uint32_t piece = a.claim;
+      while (!found) {
+        uint32_t x = piece + lane * 16u;
+        uint32_t m = 0;
+        if (lane < TAIL_PIECE / 16u) m = nl_mask16(lds_ld16(ar + at + x));
+        found = __ballot(m != 0) != 0;
+        piece += TAIL_PIECE;
+        if (found || piece >= a.claim + TAIL_MAX) break;
+        // one more piece
+        if (lane < TAIL_PIECE / 16u && t0 + piece + lane * 16u < a.nq)
+          __builtin_amdgcn_global_load_lds((const void *)(a.d + t0 + piece + lane * 16u),
+                                           (LDS_AS void *)(ar + at + piece), 16, 0, 0);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        span = piece + TAIL_PIECE;
+        if (t0 + span > a.nq) {
+          uint32_t xx = piece + lane * 16u;
+          if (lane < TAIL_PIECE / 16u && t0 + xx + 16u > a.nq)
+            lds_st16(ar + at + xx, end_patch(lds_ld16(ar + at + xx), t0 + xx, a.nq));
+          __syncthreads();
+        }
+      }
+      // ---- queue its line starts ----
+      c.t = t;
+      c.t0 = t0;
+      c.end = min(t0 + a.claim, a.nq);
+      c.at = at;
+      c.resume = t0;
+      c.rank = 0;
+      c.base = 0;
+      c.carry0 = carry0;
+      c.last_unstaged = ~0u;
+      if (!found) {
+        // the last line start of the claim: the last newline before c.end, plus one
+        uint32_t ls = 0;
+        for (uint32_t x = lane * 16u; t0 + x < c.end; x += STEP_BYTES) {
+          uint32_t m = nl_mask16(lds_ld16(ar + at + x));
+          uint32_t lim = c.end - (t0 + x);
+          if (lim < 16u) m &= (1u << lim) - 1u;
+          // a newline at the claim's last byte cannot occur here (found would be set)
+          if (m) ls = max(ls, t0 + x + 31u - (uint32_t)__builtin_clz(m) + 1u);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) ls = max(ls, (uint32_t)__shfl_xor(ls, o, 64));
+        if (t0 == 0 && ls < a.mis) ls = 0;  // newlines before the chunk start do not count
+        if (ls == 0) ls = (t0 == 0) ? a.mis : (carry0 ? t0 : ~0u);
+        c.last_unstaged = ls;
+      }
+      uint32_t got;
+      uint32_t total = enqueue_lines(a, ar, Q, qhead + qlen, QCAP - qlen, c, true, &got);
+      lines_total += total;
+      if (LB) {
+        c.base = (uint32_t)tile_lookback(a.tile_state, t, total);
+        for (uint32_t e = lane; e < got; e += 64) Q.qi[(qhead + qlen + e) & QMASK] += c.base;
+      } else if (lane == 0) {
+        a.tile_state[t] = total;
+      }
+      qlen += got;
+      pending = c.resume < c.end;
+      if (total) used = at + span;
       __syncthreads();
     }
-    if (qlen == 0) break;
+    if (qlen == 0) {
+      if (!pending && !tiles_left) break;
+      continue;
+    }
 
     // ---- one round: lane k evaluates queued line k ----
-    uint32_t take = qlen < 64 ? qlen : 64;
-    {
-      // stage the round's span [lo, hi) into LDS with coalesced 16-byte loads
-      uint32_t lo = qs[qhead & QMASK] & ~15u;
-      uint32_t hi = qlen > take ? qs[(qhead + take) & QMASK] : claim_end + 1024u;
-      uint32_t cap_hi = (a.n + 16u) & ~15u;
-      if (hi > cap_hi) hi = cap_hi;
-      if (hi > lo + STAGE_BYTES) hi = lo + STAGE_BYTES;
-      hi = (hi + 15u) & ~15u;
-      for (uint32_t off = lane * 16u; lo + off < hi; off += 1024u) stage[off >> 4] = *((const uint4 *)(a.d + lo + off));
-      rd.slo = lo;
-      rd.shi = hi;
-      __syncthreads();
-    }
+    const uint32_t take = qlen < 64 ? qlen : 64;
     if (lane < take) {
-      uint32_t e = (qhead + lane) & QMASK;
-      uint32_t s = qs[e];
-      uint32_t idx = qi[e];
-      SlotV sl[NS];
-      LineResult r = eval_line<NS>(rd, s, P, sl);
-      if (r.status == ST_OK && r.selected && P.output_kind == WARPJQ_OUT_AGG) {
-        bool has_num = false, is_int = false, undecided = false;
-        double v = 0.0;
-        int64_t iv = 0;
-        if (P.agg_slot != 0xFFFFFFFFu) {
-          SlotV x = pick<NS>(sl, P.agg_slot);
-          if (x.kind == JK_NUM) {
-            NumVal nv;
-            if (parse_number(rd, x.off, x.off + x.len, nv)) {
-              has_num = true;
-              is_int = nv.is_int;
-              iv = nv.iv;
-              v = nv.v;
-            } else {
-              undecided = true;
-            }
-          } else if (x.kind != JK_MISSING && x.kind != JK_NULL) {
-            saw_nn = true;
-          }
-        }
-        GKey gk{0, 0, 0, 0};
-        uint32_t gh = 0;
-        if (!undecided && grouped) {
-          gk = group_key(rd, pick<NS>(sl, P.group_slot));
-          if (gk.len > 0x7FFFF) undecided = true;
-          else gh = group_hash(rd, gk);
-        }
-        if (undecided) {
-          r.status = ST_FALLBACK;
-        } else {
-          if (grouped) {
-            group_line(rd, a.gt, *lgp, gk, gh, has_num, is_int, iv, v, &govf_local);
-          } else {
-            a_cnt++;
-            if (has_num) {
-              a_num++;
-              if (is_int) {
-                uint64_t lo = (uint64_t)iv;
-                uint64_t old = a_ilo;
-                a_ilo += lo;
-                a_ihi += (iv < 0 ? ~0ull : 0ull) + (a_ilo < old ? 1 : 0);
-              } else {
-                double t = a_fs + v;  // Neumaier
-                if (fabs(a_fs) >= fabs(v)) a_fc += (a_fs - t) + v;
-                else a_fc += (v - t) + a_fs;
-                a_fs = t;
-              }
-              if (v < a_mn) a_mn = v;
-              if (v > a_mx) a_mx = v;
-            }
-          }
-        }
-      }
-      if (rows) {
-        uint32_t rl = 0;
-        if (r.status == ST_OK && r.selected) {
-          uint64_t n = row<NS, DevReader, DevSink>(rd, P, sl, s, r.end, nullptr);
-          rl = (uint32_t)n;
-          c_sel++;
-        }
-        if (idx < a.max_lines) {
-          a.row_len[idx] = rl;
-          if (rl) a.sel_start[idx] = s;
-        } else {
-          ovf_local = 1;
-        }
-      }
-      switch (r.status) {
-        case ST_BLANK: c_blank++; break;
-        case ST_INVALID:
-          c_inv++;
-          atomicMin(&a.ctr->first_invalid, idx);
-          break;
-        case ST_TYPE_ERROR: c_te++; break;
-        case ST_FALLBACK: {
-          c_fb++;
-          uint32_t e = s;
-          while (rd.byte(e) != WJQ_END) e++;
-          uint32_t len = e - s;
-          if (len > 0 && rd.byte(e - 1) == '\r') len--;
-          uint32_t k = atomicAdd(&a.ctr->fb_count, 1u);
-          if (k < a.fb_cap) {
-            a.fb_idx[k] = idx;
-            a.fb_off[k] = s;
-            a.fb_len[k] = len;
-          } else {
-            ovf_local = 1;
-          }
-          if (rows && idx < a.max_lines) a.row_len[idx] = 0;
-          break;
-        }
-        default: break;
+      const uint32_t e = (qhead + lane) & QMASK;
+      const uint32_t s = Q.qs[e];
+      const uint32_t rec = Q.qi[e];
+      const uint32_t tile = Q.qt[e];
+      const uint32_t o = Q.qo[e];
+      if (!(o & Q_UNSTAGED)) {
+        LdsReader rd{ar, (o & 0x7FFFu) - s};
+        process_line<NS, GRP, LB>(rd, a, s, rec, tile, L, lgp);
+      } else {
+        DevReader rd(a.d, a.nq);
+        process_line<NS, GRP, LB>(rd, a, s, rec, tile, L, lgp);
       }
     }
     qhead = (qhead + take) & QMASK;
@@ -686,12 +911,12 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
   }
 
   // ---- flush ----
-  unsigned long long sb = wave_sum((unsigned long long)c_blank), si = wave_sum((unsigned long long)c_inv),
-                     st = wave_sum((unsigned long long)c_te), sf = wave_sum((unsigned long long)c_fb),
-                     ss = wave_sum((unsigned long long)c_sel);
-  uint64_t nn = __ballot(saw_nn);
-  uint64_t ov = __ballot(ovf_local != 0);
-  uint64_t gov = __ballot(govf_local != 0);
+  unsigned long long sb = wave_sum((unsigned long long)L.blank), si = wave_sum((unsigned long long)L.inv),
+                     st = wave_sum((unsigned long long)L.te), sf = wave_sum((unsigned long long)L.fb),
+                     ss = wave_sum((unsigned long long)L.sel);
+  uint64_t nn = __ballot(L.saw_nn);
+  uint64_t ov = __ballot(L.ovf != 0);
+  uint64_t gov = __ballot(L.govf != 0);
   if (lane == 0) {
     if (lines_total) atomicAdd(&a.ctr->n_lines, (unsigned long long)lines_total);
     if (sb) atomicAdd(&a.ctr->n_blank, sb);
@@ -703,9 +928,9 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
     if (ov) atomicOr(&a.ctr->overflow, 1u);
     if (gov) atomicOr(&a.ctr->group_overflow, 1u);
   }
-  if (P.output_kind == WARPJQ_OUT_AGG && !grouped) {
+  if (P.output_kind == WARPJQ_OUT_AGG && !GRP) {
     // exact integer part: 128-bit add across lanes
-    unsigned long long lo = a_ilo, hi = a_ihi;
+    unsigned long long lo = L.ilo, hi = L.ihi;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       unsigned long long olo = __shfl_xor(lo, o, 64), ohi = __shfl_xor(hi, o, 64);
@@ -714,23 +939,23 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
       lo = nlo;
     }
     // compensated float part: two-sum tree
-    double s = a_fs, c = a_fc;
+    double s = L.fs, cc = L.fc;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-      double os = __shfl_xor(s, o, 64), oc = __shfl_xor(c, o, 64);
+      double os = __shfl_xor(s, o, 64), oc = __shfl_xor(cc, o, 64);
       double t = s + os;
       double bp = t - s;
       double e = (s - (t - bp)) + (os - bp);
       s = t;
-      c = c + oc + e;
+      cc = cc + oc + e;
     }
-    double mn = a_mn, mx = a_mx;
+    double mn = L.mn, mx = L.mx;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       mn = fmin(mn, __shfl_xor(mn, o, 64));
       mx = fmax(mx, __shfl_xor(mx, o, 64));
     }
-    unsigned long long cnt = wave_sum((unsigned long long)a_cnt), num = wave_sum((unsigned long long)a_num);
+    unsigned long long cnt = wave_sum((unsigned long long)L.cnt), num = wave_sum((unsigned long long)L.num);
     if (lane == 0) {
       AggPart &pp = a.parts[blockIdx.x];
       pp.count = cnt;
@@ -738,7 +963,7 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
       pp.isum_lo = lo;
       pp.isum_hi = hi;
       pp.fsum = s;
-      pp.fcomp = c;
+      pp.fcomp = cc;
       pp.mn = mn;
       pp.mx = mx;
     }
@@ -746,11 +971,12 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
   if (GRP) {
     __syncthreads();
     LGroup &lg = *lgp;
+    DevReader gr(a.d, a.nq);
     for (uint32_t e = lane; e < LGROUP; e += 64) {
       unsigned long long k = lg.key[e];
       if (k == EMPTY_KEY) continue;
       GKey gk = unpack_key(k);
-      uint32_t pos = group_find(rd, a.gt, gk, lg.hash[e]);
+      uint32_t pos = group_find(gr, gr, a.gt, gk, lg.hash[e]);
       if (pos == ~0u) {
         atomicOr(&a.ctr->group_overflow, 1u);
         continue;
@@ -771,7 +997,7 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
 
 // One lane per selected row: re-evaluate and write its bytes.
 template <int NS>
-__global__ __launch_bounds__(256) void k_emit(const uint8_t *d, uint32_t n, DProg P, const DevCtr *ctr,
+__global__ __launch_bounds__(256) void k_emit(const uint8_t *d, uint32_t nq, DProg P, const DevCtr *ctr,
                                                const uint32_t *sel_idx, const uint64_t *row_off,
                                                const uint32_t *sel_start, uint8_t *out) {
   if (ctr->overflow) return;
@@ -780,7 +1006,7 @@ __global__ __launch_bounds__(256) void k_emit(const uint8_t *d, uint32_t n, DPro
        r += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t idx = sel_idx[r];
     uint32_t s = sel_start[idx];
-    DevReader rd(d, n);
+    DevReader rd(d, nq);
     SlotV sl[NS];
     LineResult lr = eval_line<NS>(rd, s, P, sl);
     DevSink w{out, row_off[r]};
@@ -789,15 +1015,15 @@ __global__ __launch_bounds__(256) void k_emit(const uint8_t *d, uint32_t n, DPro
   }
 }
 
-
 // ---------------------------------------------------------------------------
 // per-width launchers (each width is instantiated in its own translation unit)
 // ---------------------------------------------------------------------------
 
-#define WJQ_DECLARE_LAUNCHERS(NS)                                                                  \
-  void launch_main_##NS(bool grouped, uint32_t grid, hipStream_t st, const KArgs &a);             \
-  void launch_emit_##NS(uint32_t grid, hipStream_t st, const uint8_t *d, uint32_t n, const DProg &P, \
-                        const DevCtr *ctr, const uint32_t *sel_idx, const uint64_t *row_off,        \
+#define WJQ_DECLARE_LAUNCHERS(NS)                                                                        \
+  void launch_main_##NS(bool grouped, bool lb, uint32_t grid, hipStream_t st, const KArgs &a);          \
+  int occupancy_main_##NS(bool grouped, bool lb, uint32_t lds);                                         \
+  void launch_emit_##NS(uint32_t grid, hipStream_t st, const uint8_t *d, uint32_t n, const DProg &P,    \
+                        const DevCtr *ctr, const uint32_t *sel_idx, const uint64_t *row_off,           \
                         const uint32_t *sel_start, uint8_t *out);
 WJQ_DECLARE_LAUNCHERS(1)
 WJQ_DECLARE_LAUNCHERS(2)
@@ -805,14 +1031,24 @@ WJQ_DECLARE_LAUNCHERS(4)
 WJQ_DECLARE_LAUNCHERS(8)
 WJQ_DECLARE_LAUNCHERS(16)
 
-#define WJQ_DEFINE_LAUNCHERS(NS)                                                                     \
-  void launch_main_##NS(bool grouped, uint32_t grid, hipStream_t st, const KArgs &a) {              \
-    if (grouped) hipLaunchKernelGGL((k_main<NS, true>), dim3(grid), dim3(64), 0, st, a);             \
-    else hipLaunchKernelGGL((k_main<NS, false>), dim3(grid), dim3(64), 0, st, a);                    \
-  }                                                                                                  \
-  void launch_emit_##NS(uint32_t grid, hipStream_t st, const uint8_t *d, uint32_t n, const DProg &P, \
-                        const DevCtr *ctr, const uint32_t *sel_idx, const uint64_t *row_off,        \
-                        const uint32_t *sel_start, uint8_t *out) {                                  \
-    hipLaunchKernelGGL((k_emit<NS>), dim3(grid), dim3(256), 0, st, d, n, P, ctr, sel_idx, row_off,    \
-                       sel_start, out);                                                             \
+#define WJQ_DEFINE_LAUNCHERS(NS)                                                                          \
+  void launch_main_##NS(bool grouped, bool lb, uint32_t grid, hipStream_t st, const KArgs &a) {          \
+    const uint32_t lds = main_lds_bytes(a.arena);                                                         \
+    if (lb) hipLaunchKernelGGL((k_main<NS, false, true>), dim3(grid), dim3(64), lds, st, a);              \
+    else if (grouped) hipLaunchKernelGGL((k_main<NS, true, false>), dim3(grid), dim3(64), lds, st, a);    \
+    else hipLaunchKernelGGL((k_main<NS, false, false>), dim3(grid), dim3(64), lds, st, a);                \
+  }                                                                                                       \
+  int occupancy_main_##NS(bool grouped, bool lb, uint32_t lds) {                                          \
+    int b = 0;                                                                                            \
+    hipError_t e;                                                                                         \
+    if (lb) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_main<NS, false, true>, 64, lds);       \
+    else if (grouped) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_main<NS, true, false>, 64, lds); \
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_main<NS, false, false>, 64, lds);         \
+    return e == hipSuccess ? b : 0;                                                                       \
+  }                                                                                                       \
+  void launch_emit_##NS(uint32_t grid, hipStream_t st, const uint8_t *d, uint32_t n, const DProg &P,     \
+                        const DevCtr *ctr, const uint32_t *sel_idx, const uint64_t *row_off,            \
+                        const uint32_t *sel_start, uint8_t *out) {                                       \
+    hipLaunchKernelGGL((k_emit<NS>), dim3(grid), dim3(256), 0, st, d, n, P, ctr, sel_idx, row_off,         \
+                       sel_start, out);                                                                  \
   }

@@ -14,6 +14,7 @@ __global__ void k_reset(DevCtr *ctr, unsigned long long *tile_state, uint32_t nt
     ctr->n_lines = ctr->n_blank = ctr->n_invalid = ctr->n_type_error = ctr->n_fallback = 0;
     ctr->n_sel = ctr->out_total = 0;
     ctr->first_invalid = 0xFFFFFFFFu;
+    ctr->first_bad = ~0ull;
     ctr->overflow = ctr->saw_non_numeric = ctr->group_overflow = 0;
     ctr->fb_count = ctr->n_groups = ctr->ticket = ctr->tiles_done = 0;
   }
@@ -105,14 +106,14 @@ __global__ __launch_bounds__(256) void k_agg_final(const AggPart *parts, uint32_
   }
 }
 
-__global__ void k_group_compact(GroupTab g, DevCtr *ctr, warpjq_group *out, warpjq_amd_agg_ext *ext) {
+__global__ void k_group_compact(GroupTab g, DevCtr *ctr, warpjq_group *out, warpjq_amd_agg_ext *ext, uint32_t mis) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.cap; i += gridDim.x * blockDim.x) {
     unsigned long long k = g.keys[i];
     if (k == EMPTY_KEY) continue;
     GKey gk = unpack_key(k);
     uint32_t o = atomicAdd(&ctr->n_groups, 1u);
     warpjq_group r;
-    r.key_off = gk.off;
+    r.key_off = gk.kind == JK_NULL ? 0 : gk.off - mis;  // q position -> chunk offset
     r.key_len = gk.len;
     r.key_kind = gk.kind;
     r._pad = 0;
@@ -293,6 +294,48 @@ __global__ void k_row_finish(const DevCtr *ctr, uint64_t *row_off) {
 }
 
 // ---------------------------------------------------------------------------
+// aggregate queries: (claim, rank) records -> line indices in the chunk
+// ---------------------------------------------------------------------------
+
+// One block: tile_state[0..ntiles) line counts -> exclusive prefix sums.
+__global__ __launch_bounds__(1024) void k_res_prefix(unsigned long long *cnt, uint32_t ntiles) {
+  __shared__ unsigned long long wsum[16];
+  __shared__ unsigned long long run;
+  if (threadIdx.x == 0) run = 0;
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (uint32_t base = 0; base < ntiles; base += 1024) {
+    uint32_t i = base + threadIdx.x;
+    unsigned long long v = i < ntiles ? cnt[i] : 0, x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      unsigned long long y = __shfl_up(x, o, 64);
+      if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    unsigned long long p = run;
+    for (uint32_t q = 0; q < w; q++) p += wsum[q];
+    if (i < ntiles) cnt[i] = p + x - v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (uint32_t q = 0; q < 16; q++) run += wsum[q];
+    __syncthreads();
+  }
+}
+
+__global__ void k_res_records(DevCtr *ctr, const unsigned long long *prefix, uint32_t *fb_idx, const uint32_t *fb_tile,
+                              uint32_t fb_cap) {
+  uint32_t n = min(ctr->fb_count, fb_cap);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    fb_idx[k] += (uint32_t)prefix[fb_tile[k]];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && ctr->first_bad != ~0ull) {
+    unsigned long long b = ctr->first_bad;
+    ctr->first_invalid = (uint32_t)(prefix[b >> 32] + (b & 0xFFFFFFFFull));
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 
@@ -325,13 +368,17 @@ struct Slot {
   uint64_t *h_row_off = nullptr;
   uint8_t *h_out = nullptr;
 
-  uint32_t *d_fb_idx = nullptr, *d_fb_off = nullptr, *d_fb_len = nullptr;
+  uint32_t *d_fb_idx = nullptr, *d_fb_off = nullptr, *d_fb_len = nullptr, *d_fb_tile = nullptr;
+  uint32_t ntiles = 0;
   uint32_t *h_fb_idx = nullptr, *h_fb_off = nullptr, *h_fb_len = nullptr;
 
   bool submitted = false;
 };
 
 }  // namespace
+
+static uint32_t g_num_cus = 0;
+#define MAX_WG_PER_CU 32u
 
 struct warpjq_ctx {
   DProgHost host;
@@ -347,7 +394,8 @@ struct warpjq_ctx {
   uint32_t group_cap = 0;
   uint32_t grid = 0;
   uint32_t max_tiles = 0;
-  uint32_t claim = 32768;
+  uint32_t claim = CLAIM_DEFAULT;
+  uint32_t arena = ARENA_DEFAULT;
   bool grouped = false, rows = false;
   int device = 0;
 };
@@ -413,6 +461,7 @@ static void free_slot(Slot &s) {
   (void)hipFree(s.d_fb_idx);
   (void)hipFree(s.d_fb_off);
   (void)hipFree(s.d_fb_len);
+  (void)hipFree(s.d_fb_tile);
   (void)hipHostFree(s.h_fb_idx);
   (void)hipHostFree(s.h_fb_off);
   (void)hipHostFree(s.h_fb_len);
@@ -438,13 +487,14 @@ static warpjq_status alloc_slot(warpjq_ctx *ctx, Slot &s, char *err, size_t err_
   HIP_TRY(dalloc(&s.d_ctr, 1), "hipMalloc(ctr)");
   HIP_TRY(halloc(&s.h_ctr, 1), "hipHostMalloc(ctr)");
   HIP_TRY(dalloc(&s.d_tile_state, ctx->max_tiles), "hipMalloc(tile_state)");
-  HIP_TRY(dalloc(&s.d_parts, ctx->grid), "hipMalloc(parts)");
+  HIP_TRY(dalloc(&s.d_parts, (size_t)g_num_cus * MAX_WG_PER_CU), "hipMalloc(parts)");
   HIP_TRY(dalloc(&s.d_aggout, 1), "hipMalloc(agg)");
   HIP_TRY(halloc(&s.h_aggout, 1), "hipHostMalloc(agg)");
   const uint32_t fc = ctx->fb_cap;
   HIP_TRY(dalloc(&s.d_fb_idx, fc), "hipMalloc(fb)");
   HIP_TRY(dalloc(&s.d_fb_off, fc), "hipMalloc(fb)");
   HIP_TRY(dalloc(&s.d_fb_len, fc), "hipMalloc(fb)");
+  HIP_TRY(dalloc(&s.d_fb_tile, fc), "hipMalloc(fb)");
   HIP_TRY(halloc(&s.h_fb_idx, fc), "hipHostMalloc(fb)");
   HIP_TRY(halloc(&s.h_fb_off, fc), "hipHostMalloc(fb)");
   HIP_TRY(halloc(&s.h_fb_len, fc), "hipHostMalloc(fb)");
@@ -494,7 +544,23 @@ static size_t put_table(std::vector<uint8_t> &buf, const T *src, size_t n) {
   return off;
 }
 
-static uint32_t g_num_cus = 0;
+
+// Persistent grid of k_main: every workgroup resident at once.
+static uint32_t main_grid(const warpjq_ctx *ctx) {
+  const bool lb = ctx->rows;
+  const uint32_t lds = main_lds_bytes(ctx->arena);
+  int occ;
+  switch (ctx->host.ns) {
+    case 1: occ = occupancy_main_1(ctx->grouped, lb, lds); break;
+    case 2: occ = occupancy_main_2(ctx->grouped, lb, lds); break;
+    case 4: occ = occupancy_main_4(ctx->grouped, lb, lds); break;
+    case 8: occ = occupancy_main_8(ctx->grouped, lb, lds); break;
+    default: occ = occupancy_main_16(ctx->grouped, lb, lds); break;
+  }
+  if (occ < 1) occ = 1;
+  if (occ > (int)MAX_WG_PER_CU) occ = MAX_WG_PER_CU;
+  return g_num_cus * (uint32_t)occ;
+}
 
 extern "C" {
 
@@ -550,7 +616,7 @@ warpjq_status warpjq_ctx_create(const warpjq_program *prog, uint64_t max_chunk_b
     delete ctx;
     return WARPJQ_ERR_INVALID_ARG;
   }
-  hipGetDevice(&ctx->device);
+  (void)hipGetDevice(&ctx->device);
   if (!g_num_cus) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) g_num_cus = (uint32_t)prop.multiProcessorCount;
@@ -566,9 +632,9 @@ warpjq_status warpjq_ctx_create(const warpjq_program *prog, uint64_t max_chunk_b
   uint64_t g = 65536;
   while (g < max_chunk_bytes / 128 && g < (1u << 22)) g <<= 1;
   ctx->group_cap = (uint32_t)g;
-  ctx->grid = g_num_cus * 16;
+  ctx->grid = main_grid(ctx);
   // tiles for the largest chunk a slot or a resident buffer may hold
-  ctx->max_tiles = (uint32_t)((MAX_CHUNK + CLAIM_MIN - 1) / CLAIM_MIN) + 1;
+  ctx->max_tiles = (uint32_t)((MAX_CHUNK + 16 + CLAIM_MIN - 1) / CLAIM_MIN) + 1;
 
   // one device allocation holding every program table
   std::vector<uint8_t> buf;
@@ -644,15 +710,21 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const uint8_t *src, 
                                   size_t err_len) {
   hipStream_t st = s.stream;
   const uint32_t claim = ctx->claim;
-  const uint32_t ntiles = (uint32_t)((n + claim - 1) / claim);
+  const uint32_t mis = (uint32_t)((uintptr_t)src & 15u);
+  const uint8_t *base = src - mis;  // kernels address the chunk from its aligned-down base
+  const uint64_t nq = n + mis;
+  const uint32_t ntiles = n ? (uint32_t)((nq + claim - 1) / claim) : 0;
+  s.ntiles = ntiles;
   hipLaunchKernelGGL(k_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (ntiles + 255) / 256))), dim3(256),
                      0, st, s.d_ctr, s.d_tile_state, ntiles);
   if (n > 0) {
     KArgs a{};
-    a.d = src;
-    a.n = (uint32_t)n;
+    a.d = base;
+    a.mis = mis;
+    a.nq = (uint32_t)nq;
     a.ntiles = ntiles;
     a.claim = claim;
+    a.arena = ctx->arena;
     a.P = ctx->dprog;
     a.ctr = s.d_ctr;
     a.tile_state = s.d_tile_state;
@@ -664,21 +736,22 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const uint8_t *src, 
     a.fb_idx = s.d_fb_idx;
     a.fb_off = s.d_fb_off;
     a.fb_len = s.d_fb_len;
+    a.fb_tile = s.d_fb_tile;
     a.fb_cap = ctx->fb_cap;
     uint32_t grid = std::min<uint32_t>(ctx->grid, ntiles);
     HIP_TRY(hipEventRecord(s.evm0, st), "hipEventRecord");
     switch (ctx->host.ns) {
-      case 1: launch_main_1(ctx->grouped, grid, st, a); break;
-      case 2: launch_main_2(ctx->grouped, grid, st, a); break;
-      case 4: launch_main_4(ctx->grouped, grid, st, a); break;
-      case 8: launch_main_8(ctx->grouped, grid, st, a); break;
-      default: launch_main_16(ctx->grouped, grid, st, a); break;
+      case 1: launch_main_1(ctx->grouped, ctx->rows, grid, st, a); break;
+      case 2: launch_main_2(ctx->grouped, ctx->rows, grid, st, a); break;
+      case 4: launch_main_4(ctx->grouped, ctx->rows, grid, st, a); break;
+      case 8: launch_main_8(ctx->grouped, ctx->rows, grid, st, a); break;
+      default: launch_main_16(ctx->grouped, ctx->rows, grid, st, a); break;
     }
     HIP_TRY(hipGetLastError(), "k_main");
     HIP_TRY(hipEventRecord(s.evm1, st), "hipEventRecord");
     if (ctx->dprog.output_kind == WARPJQ_OUT_AGG) {
       if (ctx->grouped) {
-        hipLaunchKernelGGL(k_group_compact, dim3(1024), dim3(256), 0, st, s.gt, s.d_ctr, s.d_groups, s.d_gext);
+        hipLaunchKernelGGL(k_group_compact, dim3(1024), dim3(256), 0, st, s.gt, s.d_ctr, s.d_groups, s.d_gext, mis);
       } else {
         hipLaunchKernelGGL(k_agg_final, dim3(1), dim3(256), 0, st, s.d_parts, grid, s.d_ctr, s.d_aggout);
       }
@@ -695,11 +768,11 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const uint8_t *src, 
       uint32_t eg = std::min<uint32_t>(ctx->grid, (uint32_t)((lines_bound + 255) / 256));
       if (eg == 0) eg = 1;
       switch (ctx->host.ns) {
-        case 1: launch_emit_1(eg, st, src, (uint32_t)n, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
-        case 2: launch_emit_2(eg, st, src, (uint32_t)n, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
-        case 4: launch_emit_4(eg, st, src, (uint32_t)n, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
-        case 8: launch_emit_8(eg, st, src, (uint32_t)n, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
-        default: launch_emit_16(eg, st, src, (uint32_t)n, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        case 1: launch_emit_1(eg, st, base, (uint32_t)nq, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        case 2: launch_emit_2(eg, st, base, (uint32_t)nq, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        case 4: launch_emit_4(eg, st, base, (uint32_t)nq, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        case 8: launch_emit_8(eg, st, base, (uint32_t)nq, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        default: launch_emit_16(eg, st, base, (uint32_t)nq, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
       }
     }
     HIP_TRY(hipGetLastError(), "kernel chain");
@@ -748,7 +821,17 @@ warpjq_status warpjq_wait(warpjq_ctx *ctx, uint32_t slot, warpjq_chunk_result *o
   memset(out, 0, sizeof *out);
   out->first_invalid = 0xFFFFFFFFu;
   if (!s.submitted || s.n_bytes == 0) return WARPJQ_OK;
-  const DevCtr c = *s.h_ctr;
+  DevCtr c = *s.h_ctr;
+  if (!ctx->rows && !c.overflow && (c.first_bad != ~0ull || c.fb_count) && c.fb_count <= ctx->fb_cap) {
+    // aggregate queries recorded (claim, rank): turn them into line indices
+    hipLaunchKernelGGL(k_res_prefix, dim3(1), dim3(1024), 0, s.stream, s.d_tile_state, s.ntiles);
+    hipLaunchKernelGGL(k_res_records, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (c.fb_count + 255) / 256))),
+                       dim3(256), 0, s.stream, s.d_ctr, s.d_tile_state, s.d_fb_idx, s.d_fb_tile, ctx->fb_cap);
+    HIP_TRY(hipGetLastError(), "k_res");
+    HIP_TRY(hipMemcpyAsync(s.h_ctr, s.d_ctr, sizeof(DevCtr), hipMemcpyDeviceToHost, s.stream), "D2H(counters)");
+    HIP_TRY(hipStreamSynchronize(s.stream), "stream sync");
+    c = *s.h_ctr;
+  }
   out->n_lines = c.n_lines;
   out->n_blank = c.n_blank;
   out->n_invalid = c.n_invalid;
@@ -851,8 +934,19 @@ uint32_t warpjq_amd_kernel_width(const warpjq_ctx *ctx) { return ctx ? ctx->host
 
 warpjq_status warpjq_amd_set_claim(warpjq_ctx *ctx, uint32_t claim_bytes) {
   if (!ctx || claim_bytes < CLAIM_MIN || claim_bytes > CLAIM_MAX || claim_bytes % STEP_BYTES) return WARPJQ_ERR_INVALID_ARG;
+  if (claim_bytes + TAIL_MAX > ctx->arena) return WARPJQ_ERR_INVALID_ARG;
   ctx->claim = claim_bytes;
   return WARPJQ_OK;
 }
+
+warpjq_status warpjq_amd_set_arena(warpjq_ctx *ctx, uint32_t arena_bytes) {
+  if (!ctx || arena_bytes % 256 || arena_bytes > ARENA_MAX || arena_bytes < ctx->claim + TAIL_MAX)
+    return WARPJQ_ERR_INVALID_ARG;
+  ctx->arena = arena_bytes;
+  ctx->grid = main_grid(ctx);
+  return WARPJQ_OK;
+}
+
+uint32_t warpjq_amd_grid(const warpjq_ctx *ctx) { return ctx ? ctx->grid : 0; }
 
 }  // extern "C"
