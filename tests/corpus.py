@@ -93,6 +93,13 @@ EDGE_LINES = [
     r'{"host":"\ud83d"}', '{"nested":{"deep":{"v":1}}}', '{"arr":[]}', '{"arr":[[[[1]]]]}',
     '{"flag":false}', '{"flag":null}', '   {"status":404}   ', '{"ünïcøde key":1,"status":500}',
     '{"a":{"b":{"c":{"d":{"e":{"f":{"g":1}}}}}}}',
+    # bytes a word-at-a-time special-byte test can flag by borrow right after
+    # a real hit ('#' after '"', ']' after a backslash, ' ' after a control)
+    '{"host":"#x","status":500}', '{"#k":1,"status":500}', r'{"msg":"a\\]b","status":500}',
+    r'{"msg":"q\"#","status":500}', '{"msg":"\t ","status":500}',
+] + [
+    # the same at every offset within a 16-byte block, and numbers across block edges
+    '{"pad":"' + "p" * n + '","host":"#y","n":' + "1" * (1 + n % 17) + ',"status":500}' for n in range(16)
 ]
 
 

@@ -20,6 +20,7 @@ struct HostReader {
     for (uint32_t b = 0; b < 4; b++) v |= byte(4 * k + b) << (8 * b);
     return v;
   }
+  Blk16 block(uint32_t b) const { return Blk16{dword(4 * b), dword(4 * b + 1), dword(4 * b + 2), dword(4 * b + 3)}; }
 };
 
 struct StrSink {

@@ -106,12 +106,41 @@ WJQ_HD uint32_t str_special(uint32_t x) {
   return m & 0x80808080u;
 }
 
+// 16 bytes of the chunk, little endian dwords (the unit readers fetch).
+struct Blk16 {
+  uint32_t x, y, z, w;
+};
+
+// High bit of each of a block's 16 bytes as a 16-bit mask (bit k = byte k);
+// every byte of a..d holds 0x80 or 0x00. On gfx950 four v_dot4_u32_u8.
+WJQ_HD uint32_t movemask16(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t lo = __builtin_amdgcn_udot4(b, 0x80402010u, __builtin_amdgcn_udot4(a, 0x08040201u, 0u, false), false);
+  uint32_t hi = __builtin_amdgcn_udot4(d, 0x80402010u, __builtin_amdgcn_udot4(c, 0x08040201u, 0u, false), false);
+  return (lo >> 7) | ((hi >> 7) << 8);
+#else
+  uint32_t w[4] = {a, b, c, d}, m = 0;
+  for (int q = 0; q < 4; q++)
+    for (int k = 0; k < 4; k++) m |= ((w[q] >> (8 * k + 7)) & 1u) << (4 * q + k);
+  return m;
+#endif
+}
+
+// Bit 7 of each byte set when that byte is not an ASCII digit (exact).
+WJQ_HD uint32_t non_digit(uint32_t x) {
+  uint32_t ge30 = ((x | 0x80808080u) - 0x30303030u) & ~x;  // b < 0x80 and b >= '0'
+  uint32_t gt39 = (x & 0x7F7F7F7Fu) + 0x46464646u;          // (b & 0x7F) > '9'
+  return ~(ge30 & ~gt39) & 0x80808080u;
+}
+
 WJQ_HD int ctz32(uint32_t x) { return __builtin_ctz(x); }
+WJQ_HD int ctz64(uint64_t x) { return __builtin_ctzll(x); }
 WJQ_HD int clz64(uint64_t x) { return __builtin_clzll(x); }
 
 // ---------------------------------------------------------------------------
-// scanners. `R` provides byte(i) and dword(k) (bytes 4k..4k+3, little
-// endian); both return WJQ_END for positions at or past the chunk end.
+// scanners. `R` provides byte(i), dword(k) (bytes 4k..4k+3, little endian)
+// and block(b) (bytes 16b..16b+15); all read WJQ_END for positions at or
+// past the chunk end.
 // ---------------------------------------------------------------------------
 
 template <class R>
@@ -120,42 +149,61 @@ WJQ_HD uint32_t skip_ws(R &rd, uint32_t i) {
   return i;
 }
 
+// First position >= j flagged by str_special (16 bytes per step): the first
+// '"', '\\' or byte < 0x20 at or after j, or a byte before it that
+// str_special flags falsely after a hit earlier in the same dword (callers
+// re-check the byte). Every line ends in '\n', so it exists.
+template <class R>
+WJQ_HD uint32_t next_special(R &rd, uint32_t j) {
+  for (;;) {
+    Blk16 v = rd.block(j >> 4);
+    uint32_t m = movemask16(str_special(v.x), str_special(v.y), str_special(v.z), str_special(v.w)) &
+                 (0xFFFFu << (j & 15u));
+    if (m) return (j & ~15u) + (uint32_t)ctz32(m);
+    j = (j | 15u) + 1u;
+  }
+}
+
+// First position >= j whose byte is not an ASCII digit.
+template <class R>
+WJQ_HD uint32_t digits_end(R &rd, uint32_t j) {
+  for (;;) {
+    Blk16 v = rd.block(j >> 4);
+    uint32_t m = movemask16(non_digit(v.x), non_digit(v.y), non_digit(v.z), non_digit(v.w)) & (0xFFFFu << (j & 15u));
+    if (m) return (j & ~15u) + (uint32_t)ctz32(m);
+    j = (j | 15u) + 1u;
+  }
+}
+
 // `i` is the opening quote. On success returns true with `i` one past the
 // closing quote; `esc` is set when the body holds a backslash.
 template <class R>
 WJQ_HD bool scan_string(R &rd, uint32_t &i, bool &esc) {
   uint32_t j = i + 1;
   for (;;) {
-    uint32_t w = rd.dword(j >> 2);
-    uint32_t m = str_special(w) & ~((1u << ((j & 3) * 8)) - 1u);
-    if (m == 0) {
-      j = (j & ~3u) + 4;
-      continue;
-    }
-    uint32_t k = (uint32_t)ctz32(m) >> 3;
-    uint32_t pos = (j & ~3u) + k;
-    uint32_t c = (w >> (k * 8)) & 0xFF;
+    uint32_t pos = next_special(rd, j);
+    uint32_t c = rd.byte(pos);
     if (c == '"') {
       i = pos + 1;
       return true;
     }
-    if (c == '\\') {
-      uint32_t e = rd.byte(pos + 1);
-      if (e == '"' || e == '\\' || e == '/' || e == 'b' || e == 'f' || e == 'n' || e == 'r' || e == 't') {
-        j = pos + 2;
-      } else if (e == 'u') {
-        if (!is_hexd(rd.byte(pos + 2)) || !is_hexd(rd.byte(pos + 3)) || !is_hexd(rd.byte(pos + 4)) ||
-            !is_hexd(rd.byte(pos + 5)))
-          return false;
-        j = pos + 6;
-      } else {
-        return false;
-      }
-      esc = true;
+    if (c != '\\') {
+      if (c < 0x20) return false;  // raw control, or the line ended inside
+      j = pos + 1;                 // a byte str_special flagged by borrow
       continue;
     }
-    if (c < 0x20) return false;  // raw control, or the line ended inside
-    j = pos + 1;                  // false positive of str_special
+    uint32_t e = rd.byte(pos + 1);
+    if (e == '"' || e == '\\' || e == '/' || e == 'b' || e == 'f' || e == 'n' || e == 'r' || e == 't') {
+      j = pos + 2;
+    } else if (e == 'u') {
+      if (!is_hexd(rd.byte(pos + 2)) || !is_hexd(rd.byte(pos + 3)) || !is_hexd(rd.byte(pos + 4)) ||
+          !is_hexd(rd.byte(pos + 5)))
+        return false;
+      j = pos + 6;
+    } else {
+      return false;
+    }
+    esc = true;
   }
 }
 
@@ -164,24 +212,24 @@ template <class R>
 WJQ_HD bool scan_number(R &rd, uint32_t &i) {
   uint32_t j = i;
   if (rd.byte(j) == '-') j++;
-  uint32_t is = j;
-  uint32_t first = rd.byte(j);
-  while (is_digit(rd.byte(j))) j++;
+  const uint32_t is = j;
+  const uint32_t first = rd.byte(j);
+  j = digits_end(rd, j);
   if (j == is) return false;
   if (first == '0' && j - is > 1) return false;
-  if (rd.byte(j) == '.') {
-    j++;
-    uint32_t fs = j;
-    while (is_digit(rd.byte(j))) j++;
-    if (j == fs) return false;
-  }
   uint32_t c = rd.byte(j);
-  if (c == 'e' || c == 'E') {
+  if (c == '.') {
+    const uint32_t fs = j + 1;
+    j = digits_end(rd, fs);
+    if (j == fs) return false;
+    c = rd.byte(j);
+  }
+  if ((c | 0x20u) == 'e') {
     j++;
     c = rd.byte(j);
     if (c == '+' || c == '-') j++;
-    uint32_t es = j;
-    while (is_digit(rd.byte(j))) j++;
+    const uint32_t es = j;
+    j = digits_end(rd, es);
     if (j == es) return false;
   }
   i = j;
@@ -307,8 +355,8 @@ WJQ_HD uint32_t decoded_hash(R &rd, uint32_t a, uint32_t b, bool esc, uint32_t *
 }
 
 // Three-way compare of the decoded body [a, b) with bytes lit[0, n).
-template <class R>
-WJQ_HD int decoded_cmp(R &rd, uint32_t a, uint32_t b, bool esc, const uint8_t *lit, uint32_t n) {
+template <class R, class LP>
+WJQ_HD int decoded_cmp(R &rd, uint32_t a, uint32_t b, bool esc, LP lit, uint32_t n) {
   if (!esc) {
     uint32_t m = b - a;
     uint32_t k = m < n ? m : n;
@@ -546,18 +594,104 @@ WJQ_HD uint64_t lvl_set(uint64_t packed, uint32_t L, uint32_t v) {
 }
 
 // Does the step `st` name the object member whose raw key body is [a, b)?
-template <class R>
-WJQ_HD bool key_step_match(R &rd, const DProg &P, const warpjq_step &st, uint32_t a, uint32_t b, bool esc,
-                           uint32_t h, uint32_t dlen) {
-  if (st.kind != WARPJQ_STEP_KEY || st.key_len != dlen || st.key_hash != h) return false;
+// Raw bodies (no backslash) compare by length, then bytes.
+template <class R, class PP, class ST>
+WJQ_HD bool key_step_match(R &rd, const PP &P, const ST &st, uint32_t a, uint32_t b, bool esc) {
+  if (st.kind != WARPJQ_STEP_KEY) return false;
+  if (!esc) {
+    if (st.key_len != b - a) return false;
+    const uint32_t off = st.key_off;
+    for (uint32_t q = 0; q < b - a; q++)
+      if (rd.byte(a + q) != P.blob[off + q]) return false;
+    return true;
+  }
   return decoded_cmp(rd, a, b, esc, P.blob + st.key_off, st.key_len) == 0;
+}
+
+// Slots whose path step `L` is the key [ks, ke) (among `A`), and those for
+// which it is the last step.
+template <class R, class PP>
+WJQ_HD void key_hits(R &rd, const PP &P, uint32_t A, uint32_t L, uint32_t ks, uint32_t ke, bool esc,
+                     uint32_t &hit, uint32_t &hit_fin) {
+  hit = 0;
+  hit_fin = 0;
+  while (A) {
+    const uint32_t s = (uint32_t)ctz32(A);
+    A &= A - 1;
+    const auto &sp = P.spath[s];
+    if (key_step_match(rd, P, P.steps[sp.step_off + L], ks, ke, esc)) {
+      hit |= 1u << s;
+      if (sp.nsteps == L + 1) hit_fin |= 1u << s;
+    }
+  }
+}
+
+// Slots whose path step `L` is array index `k` (among `A`).
+template <class PP>
+WJQ_HD void index_hits(const PP &P, uint32_t A, uint32_t L, uint32_t k, uint32_t &hit, uint32_t &hit_fin) {
+  hit = 0;
+  hit_fin = 0;
+  while (A) {
+    const uint32_t s = (uint32_t)ctz32(A);
+    A &= A - 1;
+    const auto &sp = P.spath[s];
+    const auto &st = P.steps[sp.step_off + L];
+    if (st.kind == WARPJQ_STEP_INDEX && st.index == k) {
+      hit |= 1u << s;
+      if (sp.nsteps == L + 1) hit_fin |= 1u << s;
+    }
+  }
+}
+
+// A value of `kind` starts at `i` for the slots in `hit`: final ones record
+// it, intermediate ones check that it is the container their next step
+// needs (null: Missing; anything else: a type error). Returns the slots that
+// descend into it.
+template <int NS, class PP>
+WJQ_HD uint32_t value_hits(const PP &P, uint32_t hit, uint32_t hit_fin, uint32_t kind, uint32_t i, uint32_t d,
+                           SlotV (&sl)[NS], uint32_t &te) {
+  uint32_t next_act = 0;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    if (!((hit >> s) & 1)) continue;
+    if ((hit_fin >> s) & 1) {
+      sl[s].off = i;
+      sl[s].kind = kind;
+      sl[s].len = 0;
+      te &= ~(1u << s);
+    } else {
+      const auto &st = P.steps[P.spath[s].step_off + d];
+      const uint32_t want = st.kind == WARPJQ_STEP_INDEX ? (uint32_t)JK_ARR : (uint32_t)JK_OBJ;
+      sl[s].kind = JK_MISSING;
+      if (kind == JK_NULL) {
+        te &= ~(1u << s);
+      } else if (kind == want) {
+        te &= ~(1u << s);
+        next_act |= 1u << s;
+      } else {
+        te |= 1u << s;
+      }
+    }
+  }
+  return next_act;
+}
+
+template <int NS>
+WJQ_HD void close_hits(uint32_t F, uint32_t i, SlotV (&sl)[NS]) {
+#pragma unroll
+  for (int s = 0; s < NS; s++)
+    if ((F >> s) & 1) sl[s].len = i - sl[s].off;
 }
 
 // Validates the value starting at `i` and the whitespace after it up to the
 // end of the line; captures fused slots. Returns ST_OK / ST_INVALID /
 // ST_FALLBACK. On ST_OK `end` is the line terminator's position.
-template <int NS, class R>
-WJQ_HD int walk(R &rd, uint32_t i, const DProg &P, SlotV (&sl)[NS], uint32_t &te, uint32_t &end) {
+//
+// One loop iteration per object member (key, ':', value, then ',' or the
+// close) or per array element, so a flat object costs one iteration per
+// member. Containers open by falling into the VALUE state.
+template <int NS, class R, class PP>
+WJQ_HD int walk(R &rd, uint32_t i, const PP &P, SlotV (&sl)[NS], uint32_t &te, uint32_t &end) {
   uint64_t arr = 0;  // bit k: the container at depth k is an array
   uint32_t d = 0;    // open containers
   uint64_t act = 0;  // live slots per container level 0..3
@@ -565,55 +699,98 @@ WJQ_HD int walk(R &rd, uint32_t i, const DProg &P, SlotV (&sl)[NS], uint32_t &te
   uint64_t cnt = 0;  // element index per array level 0..3
   uint32_t hit = P.fused_mask;  // slots for which the next value matters
   uint32_t hit_fin = 0;         // ... as the path's final value
-  enum { VALUE, KEY, AFTER } state = VALUE;
+  enum { VALUE, MEMBER, AFTER } state = VALUE;
+  i = skip_ws(rd, i);
+  uint32_t c = rd.byte(i);  // the byte at i, not whitespace, in VALUE and MEMBER
   for (;;) {
-    if (state == VALUE) {
-      i = skip_ws(rd, i);
-      uint32_t c = rd.byte(i);
+    if (state == MEMBER) {
+      // ---- "key": value, at i (c is its first byte) ----
+      if (c != '"') return ST_INVALID;
+      const uint32_t ks = i + 1;
+      bool esc = false;
+      if (!scan_string(rd, i, esc)) return ST_INVALID;
+      const uint32_t L = d - 1;
+      const uint32_t A = L < 4 ? lvl_get(act, L) : 0u;
+      hit = 0;
+      hit_fin = 0;
+      if (A) key_hits(rd, P, A, L, ks, i - 1, esc, hit, hit_fin);
+      c = rd.byte(i);
+      if (is_jws(c)) {
+        i = skip_ws(rd, i);
+        c = rd.byte(i);
+      }
+      if (c != ':') return ST_INVALID;
+      i++;
+      c = rd.byte(i);
+      if (is_jws(c)) {
+        i = skip_ws(rd, i);
+        c = rd.byte(i);
+      }
+      if (c == '{' || c == '[') {
+        state = VALUE;
+        continue;
+      }
+      // scalar member value
       uint32_t kind;
-      if (c == '{') kind = JK_OBJ;
+      const uint32_t j0 = i;
+      bool ok;
+      if (c == '"') {
+        kind = JK_STR;
+        bool e2 = false;
+        ok = scan_string(rd, i, e2);
+      } else if (c == '-' || is_digit(c)) {
+        kind = JK_NUM;
+        ok = scan_number(rd, i);
+      } else if (c == 't' || c == 'f' || c == 'n') {
+        kind = c == 'n' ? (uint32_t)JK_NULL : (uint32_t)JK_BOOL;
+        ok = scan_lit(rd, i, c);
+      } else {
+        return ST_INVALID;
+      }
+      if (!ok) return ST_INVALID;
+      if (hit) {
+        value_hits<NS>(P, hit, hit_fin, kind, j0, d, sl, te);
+        close_hits<NS>(hit_fin, i, sl);
+      }
+      c = rd.byte(i);
+      if (is_jws(c)) {
+        i = skip_ws(rd, i);
+        c = rd.byte(i);
+      }
+      if (c == ',') {
+        i++;
+        c = rd.byte(i);
+        if (is_jws(c)) {
+          i = skip_ws(rd, i);
+          c = rd.byte(i);
+        }
+        continue;  // next member
+      }
+      state = AFTER;
+      continue;
+    }
+    if (state == VALUE) {
+      // ---- a value at i (c is its first byte) ----
+      uint32_t kind;
+      if (c == '"') kind = JK_STR;
+      else if (c == '-' || is_digit(c)) kind = JK_NUM;
+      else if (c == '{') kind = JK_OBJ;
       else if (c == '[') kind = JK_ARR;
-      else if (c == '"') kind = JK_STR;
       else if (c == 't' || c == 'f') kind = JK_BOOL;
       else if (c == 'n') kind = JK_NULL;
-      else if (c == '-' || is_digit(c)) kind = JK_NUM;
       else return ST_INVALID;
-      uint32_t next_act = 0;
-      if (hit) {
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-          if (!((hit >> s) & 1)) continue;
-          if ((hit_fin >> s) & 1) {
-            sl[s].off = i;
-            sl[s].kind = kind;
-            sl[s].len = 0;
-            te &= ~(1u << s);
-          } else {
-            // intermediate value: the next step of this slot is step d
-            const warpjq_step &st = P.steps[P.spath[s].step_off + d];
-            uint32_t want = st.kind == WARPJQ_STEP_INDEX ? (uint32_t)JK_ARR : (uint32_t)JK_OBJ;
-            sl[s].kind = JK_MISSING;
-            if (kind == JK_NULL) {
-              te &= ~(1u << s);
-            } else if (kind == want) {
-              te &= ~(1u << s);
-              next_act |= 1u << s;
-            } else {
-              te |= 1u << s;
-            }
-          }
-        }
-      }
+      const uint32_t next_act = hit ? value_hits<NS>(P, hit, hit_fin, kind, i, d, sl, te) : 0u;
       if (kind == JK_OBJ || kind == JK_ARR) {
-        uint32_t close = kind == JK_OBJ ? '}' : ']';
-        uint32_t j = skip_ws(rd, i + 1);
-        if (rd.byte(j) == close) {
+        const uint32_t close = kind == JK_OBJ ? '}' : ']';
+        uint32_t j = i + 1;
+        c = rd.byte(j);
+        if (is_jws(c)) {
+          j = skip_ws(rd, j);
+          c = rd.byte(j);
+        }
+        if (c == close) {
           i = j + 1;
-          if (hit_fin) {
-#pragma unroll
-            for (int s = 0; s < NS; s++)
-              if ((hit_fin >> s) & 1) sl[s].len = i - sl[s].off;
-          }
+          if (hit_fin) close_hits<NS>(hit_fin, i, sl);
           state = AFTER;
           continue;
         }
@@ -628,121 +805,73 @@ WJQ_HD int walk(R &rd, uint32_t i, const DProg &P, SlotV (&sl)[NS], uint32_t &te
         d++;
         i = j;
         if (kind == JK_OBJ) {
-          state = KEY;
+          state = MEMBER;
         } else {
-          // first element, index 0
+          const uint32_t L = d - 1;
+          const uint32_t A = L < 4 ? lvl_get(act, L) : 0u;
           hit = 0;
           hit_fin = 0;
-          uint32_t L = d - 1;
-          uint32_t A = L < 4 ? lvl_get(act, L) : 0;
-          if (A) {
-#pragma unroll
-            for (int s = 0; s < NS; s++) {
-              if (!((A >> s) & 1)) continue;
-              const warpjq_step &st = P.steps[P.spath[s].step_off + L];
-              if (st.kind == WARPJQ_STEP_INDEX && st.index == 0) {
-                hit |= 1u << s;
-                if (P.spath[s].nsteps == L + 1) hit_fin |= 1u << s;
-              }
-            }
-          }
-          state = VALUE;
+          if (A) index_hits(P, A, L, 0, hit, hit_fin);
         }
         continue;
       }
-      uint32_t j = i;
+      const uint32_t j0 = i;
       bool ok;
       if (kind == JK_STR) {
         bool esc = false;
-        ok = scan_string(rd, j, esc);
+        ok = scan_string(rd, i, esc);
       } else if (kind == JK_NUM) {
-        ok = scan_number(rd, j);
+        ok = scan_number(rd, i);
       } else {
-        ok = scan_lit(rd, j, c);
+        ok = scan_lit(rd, i, c);
       }
       if (!ok) return ST_INVALID;
-      if (hit_fin) {
-#pragma unroll
-        for (int s = 0; s < NS; s++)
-          if ((hit_fin >> s) & 1) sl[s].len = j - i;
-      }
-      i = j;
+      if (hit_fin) close_hits<NS>(hit_fin, i, sl);
+      (void)j0;
       state = AFTER;
-    } else if (state == KEY) {
+      continue;
+    }
+    // ---- AFTER: a value ended at i; ',' or the container's close ----
+    if (d == 0) break;
+    c = rd.byte(i);
+    if (is_jws(c)) {
       i = skip_ws(rd, i);
-      if (rd.byte(i) != '"') return ST_INVALID;
-      uint32_t L = d - 1;
-      uint32_t A = L < 4 ? lvl_get(act, L) : 0;
-      uint32_t ks = i + 1;
-      bool esc = false;
-      if (!scan_string(rd, i, esc)) return ST_INVALID;
-      uint32_t ke = i - 1;
-      hit = 0;
-      hit_fin = 0;
-      if (A) {
-        uint32_t dlen;
-        uint32_t h = decoded_hash(rd, ks, ke, esc, &dlen);
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-          if (!((A >> s) & 1)) continue;
-          const warpjq_step &st = P.steps[P.spath[s].step_off + L];
-          if (key_step_match(rd, P, st, ks, ke, esc, h, dlen)) {
-            hit |= 1u << s;
-            if (P.spath[s].nsteps == L + 1) hit_fin |= 1u << s;
-          }
-        }
-      }
-      i = skip_ws(rd, i);
-      if (rd.byte(i) != ':') return ST_INVALID;
+      c = rd.byte(i);
+    }
+    const bool top_arr = (arr >> (d - 1)) & 1;
+    if (c == ',') {
       i++;
-      state = VALUE;
-    } else {  // AFTER
-      if (d == 0) break;
-      i = skip_ws(rd, i);
-      uint32_t c = rd.byte(i);
-      bool top_arr = (arr >> (d - 1)) & 1;
-      if (c == ',') {
-        i++;
-        if (top_arr) {
-          hit = 0;
-          hit_fin = 0;
-          uint32_t L = d - 1;
-          uint32_t A = L < 4 ? lvl_get(act, L) : 0;
-          if (A) {
-            uint32_t k = lvl_get(cnt, L);
-            if (k < 0xFFFF) k++;
-            cnt = lvl_set(cnt, L, k);
-#pragma unroll
-            for (int s = 0; s < NS; s++) {
-              if (!((A >> s) & 1)) continue;
-              const warpjq_step &st = P.steps[P.spath[s].step_off + L];
-              if (st.kind == WARPJQ_STEP_INDEX && st.index == k) {
-                hit |= 1u << s;
-                if (P.spath[s].nsteps == L + 1) hit_fin |= 1u << s;
-              }
-            }
-          }
-          state = VALUE;
-        } else {
-          state = KEY;
-        }
-      } else if ((c == '}' && !top_arr) || (c == ']' && top_arr)) {
-        i++;
-        d--;
-        if (d < 4) {
-          uint32_t F = lvl_get(fin, d);
-          if (F) {
-#pragma unroll
-            for (int s = 0; s < NS; s++)
-              if ((F >> s) & 1) sl[s].len = i - sl[s].off;
-          }
-          act = lvl_set(act, d, 0);
-          fin = lvl_set(fin, d, 0);
-        }
-        state = AFTER;
-      } else {
-        return ST_INVALID;
+      c = rd.byte(i);
+      if (is_jws(c)) {
+        i = skip_ws(rd, i);
+        c = rd.byte(i);
       }
+      if (top_arr) {
+        const uint32_t L = d - 1;
+        const uint32_t A = L < 4 ? lvl_get(act, L) : 0u;
+        hit = 0;
+        hit_fin = 0;
+        if (A) {
+          uint32_t k = lvl_get(cnt, L);
+          if (k < 0xFFFF) k++;
+          cnt = lvl_set(cnt, L, k);
+          index_hits(P, A, L, k, hit, hit_fin);
+        }
+        state = VALUE;
+      } else {
+        state = MEMBER;
+      }
+    } else if (c == (top_arr ? ']' : '}')) {
+      i++;
+      d--;
+      if (d < 4) {
+        const uint32_t F = lvl_get(fin, d);
+        if (F) close_hits<NS>(F, i, sl);
+        act = lvl_set(act, d, 0);
+        fin = lvl_set(fin, d, 0);
+      }
+    } else {
+      return ST_INVALID;
     }
   }
   i = skip_ws(rd, i);
@@ -787,8 +916,8 @@ WJQ_HD uint32_t skip_valid(R &rd, uint32_t i) {
 
 // Sequential lookup (json.rs:444-508) on a validated line, for paths longer
 // than the walk captures. Returns false for a type error.
-template <class R>
-WJQ_HD bool lookup_valid(R &rd, uint32_t line, const DProg &P, const SlotPath &sp, SlotV &out) {
+template <class R, class PP, class SP>
+WJQ_HD bool lookup_valid(R &rd, uint32_t line, const PP &P, const SP &sp, SlotV &out) {
   uint32_t start = skip_ws(rd, line);
   uint32_t kind = kind_of(rd.byte(start));
   uint32_t endp = 0;
@@ -797,7 +926,7 @@ WJQ_HD bool lookup_valid(R &rd, uint32_t line, const DProg &P, const SlotPath &s
       out.kind = JK_MISSING;
       return true;
     }
-    const warpjq_step &st = P.steps[sp.step_off + k];
+    const auto &st = P.steps[sp.step_off + k];
     bool found = false;
     uint32_t vs = 0, ve = 0;
     if (st.kind == WARPJQ_STEP_KEY) {
@@ -882,8 +1011,8 @@ WJQ_HD uint32_t slot_rank(R &rd, const SlotV &s) {
 }
 
 // Returns -1/0/1, or 2 when a number could not be decided.
-template <class R>
-WJQ_HD int compare_lit(R &rd, const DProg &P, const SlotV &s, const warpjq_cmp &c) {
+template <class R, class PP, class CC>
+WJQ_HD int compare_lit(R &rd, const PP &P, const SlotV &s, const CC &c) {
   uint32_t lrank = c.lit_kind;  // NULL 0, FALSE 1, TRUE 2, NUM 3, STR 4
   uint32_t srank = slot_rank(rd, s);
   if (srank != lrank) return srank < lrank ? -1 : 1;
@@ -919,14 +1048,14 @@ WJQ_HD bool truthy(R &rd, const SlotV &s) {
 }
 
 // Evaluates the RPN filter. Returns 0/1, or 2 when undecided.
-template <int NS, class R>
-WJQ_HD int eval_cond(R &rd, const DProg &P, const SlotV (&sl)[NS]) {
+template <int NS, class R, class PP>
+WJQ_HD int eval_cond(R &rd, const PP &P, const SlotV (&sl)[NS]) {
   uint32_t stack = 0, sp = 0;
   for (uint32_t k = 0; k < P.n_cond; k++) {
-    const warpjq_cond_op o = P.cond[k];
+    const auto &o = P.cond[k];
     uint32_t v;
     if (o.op == WARPJQ_COND_CMP) {
-      const warpjq_cmp &c = P.cmps[o.arg];
+      const auto &c = P.cmps[o.arg];
       uint32_t s = P.slot_of_path[c.path];
       SlotV x = sl[0];
 #pragma unroll
@@ -966,8 +1095,8 @@ struct LineResult {
   uint32_t end;       // position of the terminator (valid lines)
 };
 
-template <int NS, class R>
-WJQ_HD LineResult eval_line(R &rd, uint32_t s, const DProg &P, SlotV (&sl)[NS]) {
+template <int NS, class R, class PP>
+WJQ_HD LineResult eval_line(R &rd, uint32_t s, const PP &P, SlotV (&sl)[NS]) {
   LineResult r{ST_OK, 0, 0};
   uint32_t i = s;
   bool ff = false;
@@ -1087,8 +1216,8 @@ WJQ_HD uint64_t csv_cell(R &rd, const SlotV &s, W *w) {
   return len;
 }
 
-template <int NS, class R, class W>
-WJQ_HD uint64_t put_value(R &rd, const DProg &P, const SlotV &s, W *w) {
+template <int NS, class R, class W, class PP>
+WJQ_HD uint64_t put_value(R &rd, const PP &P, const SlotV &s, W *w) {
   if (P.csv_mode) return csv_cell(rd, s, w);
   if (s.kind == JK_MISSING) {
     if (w) {
@@ -1113,8 +1242,8 @@ WJQ_HD SlotV pick(const SlotV (&sl)[NS], uint32_t s) {
 }
 
 // Row for a selected line starting at `line` with terminator at `end`.
-template <int NS, class R, class W>
-WJQ_HD uint64_t row(R &rd, const DProg &P, const SlotV (&sl)[NS], uint32_t line, uint32_t end, W *w) {
+template <int NS, class R, class W, class PP>
+WJQ_HD uint64_t row(R &rd, const PP &P, const SlotV (&sl)[NS], uint32_t line, uint32_t end, W *w) {
   uint64_t n = 0;
   if (P.output_kind == WARPJQ_OUT_PASSTHROUGH) {
     uint32_t e = end;

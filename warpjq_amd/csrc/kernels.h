@@ -80,6 +80,7 @@ using namespace wjqd;
 #define MAX_CHUNK (((uint64_t)1 << 32) - ((uint64_t)1 << 20))
 #define ROWSCAN_BLOCK 1024u          // lines per block in the row-offset scan
 #define EMPTY_KEY (~0ull)
+#define TAB_LDS_MAX 16384u           // program tables staged in LDS per workgroup
 
 // ---------------------------------------------------------------------------
 // device-side structures
@@ -116,6 +117,8 @@ struct KArgs {
   uint32_t claim;    // bytes per claim (multiple of STEP_BYTES)
   uint32_t arena;    // LDS arena bytes per wave (multiple of 256)
   DProg P;
+  const uint8_t *tab;  // the program tables P points into (copied to LDS per workgroup)
+  uint32_t tab_bytes;
   DevCtr *ctr;
   // row outputs: look-back states; aggregates: line count per claim
   unsigned long long *tile_state;
@@ -140,51 +143,118 @@ struct DevReader {
   const uint8_t *d;
   uint32_t n;
   uint32_t cb;
-  uint32_t v0, v1, v2, v3;
+  Blk16 v;
 
-  __device__ DevReader(const uint8_t *d_, uint32_t n_) : d(d_), n(n_), cb(0xFFFFFFFFu), v0(0), v1(0), v2(0), v3(0) {}
+  __device__ DevReader(const uint8_t *d_, uint32_t n_) : d(d_), n(n_), cb(0xFFFFFFFFu), v{0, 0, 0, 0} {}
 
-  __device__ __forceinline__ uint32_t dword(uint32_t k) {
-    uint32_t b = k >> 2;
+  __device__ __forceinline__ Blk16 block(uint32_t b) {
     if (b != cb) {
-      uint4 v = *((const uint4 *)d + b);
-      v0 = v.x;
-      v1 = v.y;
-      v2 = v.z;
-      v3 = v.w;
+      uint4 x = *((const uint4 *)d + b);
+      v = Blk16{x.x, x.y, x.z, x.w};
+      uint32_t p = b << 4;
+      if (p + 15 >= n) {
+        uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (uint32_t q = 0; q < 16; q++)
+          if (p + q >= n) w[q >> 2] = (w[q >> 2] & ~(0xFFu << (8 * (q & 3)))) | (WJQ_END << (8 * (q & 3)));
+        v = Blk16{w[0], w[1], w[2], w[3]};
+      }
       cb = b;
     }
+    return v;
+  }
+  __device__ __forceinline__ uint32_t dword(uint32_t k) {
+    Blk16 x = block(k >> 2);
     uint32_t s = k & 3;
-    uint32_t x = s == 0 ? v0 : s == 1 ? v1 : s == 2 ? v2 : v3;
-    uint32_t p = k << 2;
-    if (p + 3 >= n) {
-#pragma unroll
-      for (uint32_t q = 0; q < 4; q++)
-        if (p + q >= n) x = (x & ~(0xFFu << (8 * q))) | (WJQ_END << (8 * q));
-    }
-    return x;
+    return s == 0 ? x.x : s == 1 ? x.y : s == 2 ? x.z : x.w;
   }
-  __device__ __forceinline__ uint32_t byte(uint32_t i) {
-    if (i >= n) return WJQ_END;
-    return (dword(i >> 2) >> ((i & 3) * 8)) & 0xFFu;
-  }
+  __device__ __forceinline__ uint32_t byte(uint32_t i) { return (dword(i >> 2) >> ((i & 3) * 8)) & 0xFFu; }
 };
 
 // Reads a line staged in the wave's LDS arena: position i lives at arena
-// byte i + delta. Callers only read up to the line's terminator, which is
-// staged (bytes past the chunk end are staged as '\n').
+// byte i + delta (delta is a multiple of 16, so blocks are aligned). Caches
+// one 16-byte block in registers. Callers only read up to the line's
+// terminator, which is staged (bytes past the chunk end are staged as '\n').
+typedef unsigned int wjq_v4u __attribute__((ext_vector_type(4)));
 struct LdsReader {
   const LDS_AS uint8_t *b;
   uint32_t delta;
-  __device__ __forceinline__ uint32_t byte(uint32_t i) const { return b[i + delta]; }
-  __device__ __forceinline__ uint32_t dword(uint32_t k) const {
-    return *(const LDS_AS uint32_t *)(b + ((k << 2) + delta));
+  uint32_t cb;
+  Blk16 v;
+
+  __device__ LdsReader(const LDS_AS uint8_t *b_, uint32_t delta_) : b(b_), delta(delta_), cb(0xFFFFFFFFu), v{0, 0, 0, 0} {}
+
+  __device__ __forceinline__ Blk16 block(uint32_t k) {
+    if (k != cb) {
+      wjq_v4u x = *(const LDS_AS wjq_v4u *)(b + ((k << 4) + delta));
+      v = Blk16{x.x, x.y, x.z, x.w};
+      cb = k;
+    }
+    return v;
   }
+  __device__ __forceinline__ uint32_t dword(uint32_t k) {
+    Blk16 x = block(k >> 2);
+    uint32_t s = k & 3;
+    return s == 0 ? x.x : s == 1 ? x.y : s == 2 ? x.z : x.w;
+  }
+  __device__ __forceinline__ uint32_t byte(uint32_t i) { return (dword(i >> 2) >> ((i & 3) * 8)) & 0xFFu; }
 };
+
+// The program tables staged in LDS: the same fields as DProg, read with
+// ds_read through address-space-3 pointers (lane-varying indices into the
+// step table would otherwise be dependent HBM loads on every key).
+struct LProg {
+  const LDS_AS warpjq_step *steps;
+  const LDS_AS warpjq_cmp *cmps;
+  const LDS_AS warpjq_cond_op *cond;
+  const LDS_AS uint8_t *blob;
+  const LDS_AS SlotPath *spath;
+  const LDS_AS uint32_t *slot_of_path;
+  const LDS_AS uint32_t *field_slot;
+  const LDS_AS uint32_t *prefix_off;
+  const LDS_AS uint32_t *prefix_len;
+  uint32_t n_slots, n_cond, fused_mask, filter_mask, post_mask, has_filter, output_kind, output_slot, agg_kind,
+      agg_slot, group_slot, n_fields, suffix_off, suffix_len, csv_mode;
+};
+
+// Copies the table block into LDS (whole workgroup) and rebases P onto it.
+__device__ __forceinline__ LProg stage_prog(const DProg &P, const uint8_t *tab, uint32_t tab_bytes,
+                                            LDS_AS uint8_t *lds) {
+  for (uint32_t o = threadIdx.x * 4u; o < tab_bytes; o += blockDim.x * 4u)
+    *(LDS_AS uint32_t *)(lds + o) = *(const uint32_t *)(tab + o);
+  __syncthreads();
+  LProg L;
+#define WJQ_REBASE(f, T) L.f = (const LDS_AS T *)(lds + (uint32_t)((const uint8_t *)P.f - tab))
+  WJQ_REBASE(steps, warpjq_step);
+  WJQ_REBASE(cmps, warpjq_cmp);
+  WJQ_REBASE(cond, warpjq_cond_op);
+  WJQ_REBASE(blob, uint8_t);
+  WJQ_REBASE(spath, SlotPath);
+  WJQ_REBASE(slot_of_path, uint32_t);
+  WJQ_REBASE(field_slot, uint32_t);
+  WJQ_REBASE(prefix_off, uint32_t);
+  WJQ_REBASE(prefix_len, uint32_t);
+#undef WJQ_REBASE
+  L.n_slots = P.n_slots;
+  L.n_cond = P.n_cond;
+  L.fused_mask = P.fused_mask;
+  L.filter_mask = P.filter_mask;
+  L.post_mask = P.post_mask;
+  L.has_filter = P.has_filter;
+  L.output_kind = P.output_kind;
+  L.output_slot = P.output_slot;
+  L.agg_kind = P.agg_kind;
+  L.agg_slot = P.agg_slot;
+  L.group_slot = P.group_slot;
+  L.n_fields = P.n_fields;
+  L.suffix_off = P.suffix_off;
+  L.suffix_len = P.suffix_len;
+  L.csv_mode = P.csv_mode;
+  return L;
+}
 
 // 16-byte LDS accesses through address-space-3 pointers (ds_read_b128 /
 // ds_write_b128).
-typedef unsigned int wjq_v4u __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ uint4 lds_ld16(const LDS_AS uint8_t *p) {
   wjq_v4u v = *(const LDS_AS wjq_v4u *)p;
   return make_uint4(v.x, v.y, v.z, v.w);
@@ -492,9 +562,8 @@ struct LaneAcc {
 // group table or the per-line row arrays. `rec` is the line's index in the
 // chunk (row outputs) or its rank in claim `tile` (aggregates).
 template <int NS, bool GRP, bool LB, class R>
-__device__ __forceinline__ void process_line(R &rd, const KArgs &a, uint32_t s, uint32_t rec, uint32_t tile,
-                                             LaneAcc &L, LGroup *lgp) {
-  const DProg &P = a.P;
+__device__ __forceinline__ void process_line(R &rd, const KArgs &a, const LProg &P, uint32_t s, uint32_t rec,
+                                             uint32_t tile, LaneAcc &L, LGroup *lgp) {
   SlotV sl[NS];
   LineResult r = eval_line<NS>(rd, s, P, sl);
   if (r.status == ST_OK && r.selected && P.output_kind == WARPJQ_OUT_AGG) {
@@ -611,8 +680,10 @@ struct LGroupStore<false> {
 
 extern __shared__ uint4 wjq_dyn_lds[];
 
-// Dynamic LDS: [arena | qs | qi | qt | qo].
-__host__ __device__ constexpr uint32_t main_lds_bytes(uint32_t arena) { return arena + QCAP * (4 + 4 + 4 + 2); }
+// Dynamic LDS: [arena | qs | qi | qt | qo | program tables].
+__host__ __device__ constexpr uint32_t main_lds_bytes(uint32_t arena, uint32_t tab_bytes) {
+  return arena + QCAP * (4 + 4 + 4 + 2) + ((tab_bytes + 15u) & ~15u);
+}
 
 // The queue of line starts (an LDS ring of QCAP entries): q position, index
 // or rank, claim, arena offset (bit 15: the line is not wholly staged).
@@ -721,9 +792,9 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
   Q.qt = Q.qi + QCAP;
   Q.qo = (LDS_AS uint16_t *)(Q.qt + QCAP);
   LGroup *lgp = LGroupStore<GRP>::get();
+  const LProg P = stage_prog(a.P, a.tab, a.tab_bytes, ar + main_lds_bytes(a.arena, 0));
 
   const uint32_t lane = lane_id();
-  const DProg &P = a.P;
 
   if (GRP) {
     LGroup &lg = *lgp;
@@ -898,11 +969,11 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
       const uint32_t tile = Q.qt[e];
       const uint32_t o = Q.qo[e];
       if (!(o & Q_UNSTAGED)) {
-        LdsReader rd{ar, (o & 0x7FFFu) - s};
-        process_line<NS, GRP, LB>(rd, a, s, rec, tile, L, lgp);
+        LdsReader rd(ar, (o & 0x7FFFu) - s);
+        process_line<NS, GRP, LB>(rd, a, P, s, rec, tile, L, lgp);
       } else {
         DevReader rd(a.d, a.nq);
-        process_line<NS, GRP, LB>(rd, a, s, rec, tile, L, lgp);
+        process_line<NS, GRP, LB>(rd, a, P, s, rec, tile, L, lgp);
       }
     }
     qhead = (qhead + take) & QMASK;
@@ -997,9 +1068,10 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
 
 // One lane per selected row: re-evaluate and write its bytes.
 template <int NS>
-__global__ __launch_bounds__(256) void k_emit(const uint8_t *d, uint32_t nq, DProg P, const DevCtr *ctr,
-                                               const uint32_t *sel_idx, const uint64_t *row_off,
-                                               const uint32_t *sel_start, uint8_t *out) {
+__global__ __launch_bounds__(256) void k_emit(const uint8_t *d, uint32_t nq, DProg P0, const uint8_t *tab,
+                                               uint32_t tab_bytes, const DevCtr *ctr, const uint32_t *sel_idx,
+                                               const uint64_t *row_off, const uint32_t *sel_start, uint8_t *out) {
+  const LProg P = stage_prog(P0, tab, tab_bytes, (LDS_AS uint8_t *)wjq_dyn_lds);
   if (ctr->overflow) return;
   uint64_t nsel = ctr->n_sel;
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nsel;
@@ -1023,8 +1095,8 @@ __global__ __launch_bounds__(256) void k_emit(const uint8_t *d, uint32_t nq, DPr
   void launch_main_##NS(bool grouped, bool lb, uint32_t grid, hipStream_t st, const KArgs &a);          \
   int occupancy_main_##NS(bool grouped, bool lb, uint32_t lds);                                         \
   void launch_emit_##NS(uint32_t grid, hipStream_t st, const uint8_t *d, uint32_t n, const DProg &P,    \
-                        const DevCtr *ctr, const uint32_t *sel_idx, const uint64_t *row_off,           \
-                        const uint32_t *sel_start, uint8_t *out);
+                        const uint8_t *tab, uint32_t tab_bytes, const DevCtr *ctr, const uint32_t *sel_idx, \
+                        const uint64_t *row_off, const uint32_t *sel_start, uint8_t *out);
 WJQ_DECLARE_LAUNCHERS(1)
 WJQ_DECLARE_LAUNCHERS(2)
 WJQ_DECLARE_LAUNCHERS(4)
@@ -1033,7 +1105,7 @@ WJQ_DECLARE_LAUNCHERS(16)
 
 #define WJQ_DEFINE_LAUNCHERS(NS)                                                                          \
   void launch_main_##NS(bool grouped, bool lb, uint32_t grid, hipStream_t st, const KArgs &a) {          \
-    const uint32_t lds = main_lds_bytes(a.arena);                                                         \
+    const uint32_t lds = main_lds_bytes(a.arena, a.tab_bytes);                                            \
     if (lb) hipLaunchKernelGGL((k_main<NS, false, true>), dim3(grid), dim3(64), lds, st, a);              \
     else if (grouped) hipLaunchKernelGGL((k_main<NS, true, false>), dim3(grid), dim3(64), lds, st, a);    \
     else hipLaunchKernelGGL((k_main<NS, false, false>), dim3(grid), dim3(64), lds, st, a);                \
@@ -1047,8 +1119,8 @@ WJQ_DECLARE_LAUNCHERS(16)
     return e == hipSuccess ? b : 0;                                                                       \
   }                                                                                                       \
   void launch_emit_##NS(uint32_t grid, hipStream_t st, const uint8_t *d, uint32_t n, const DProg &P,     \
-                        const DevCtr *ctr, const uint32_t *sel_idx, const uint64_t *row_off,            \
-                        const uint32_t *sel_start, uint8_t *out) {                                       \
-    hipLaunchKernelGGL((k_emit<NS>), dim3(grid), dim3(256), 0, st, d, n, P, ctr, sel_idx, row_off,         \
-                       sel_start, out);                                                                  \
+                        const uint8_t *tab, uint32_t tab_bytes, const DevCtr *ctr, const uint32_t *sel_idx, \
+                        const uint64_t *row_off, const uint32_t *sel_start, uint8_t *out) {              \
+    hipLaunchKernelGGL((k_emit<NS>), dim3(grid), dim3(256), (tab_bytes + 15u) & ~15u, st, d, n, P, tab,     \
+                       tab_bytes, ctr, sel_idx, row_off, sel_start, out);                                \
   }

@@ -384,6 +384,7 @@ struct warpjq_ctx {
   DProgHost host;
   // device copies of the program tables
   void *d_tables = nullptr;
+  uint32_t tab_bytes = 0;
   DProg dprog{};
   Slot *slots = nullptr;
   uint32_t n_slots = 0;
@@ -548,7 +549,7 @@ static size_t put_table(std::vector<uint8_t> &buf, const T *src, size_t n) {
 // Persistent grid of k_main: every workgroup resident at once.
 static uint32_t main_grid(const warpjq_ctx *ctx) {
   const bool lb = ctx->rows;
-  const uint32_t lds = main_lds_bytes(ctx->arena);
+  const uint32_t lds = main_lds_bytes(ctx->arena, ctx->tab_bytes);
   int occ;
   switch (ctx->host.ns) {
     case 1: occ = occupancy_main_1(ctx->grouped, lb, lds); break;
@@ -632,7 +633,6 @@ warpjq_status warpjq_ctx_create(const warpjq_program *prog, uint64_t max_chunk_b
   uint64_t g = 65536;
   while (g < max_chunk_bytes / 128 && g < (1u << 22)) g <<= 1;
   ctx->group_cap = (uint32_t)g;
-  ctx->grid = main_grid(ctx);
   // tiles for the largest chunk a slot or a resident buffer may hold
   ctx->max_tiles = (uint32_t)((MAX_CHUNK + 16 + CLAIM_MIN - 1) / CLAIM_MIN) + 1;
 
@@ -648,6 +648,14 @@ warpjq_status warpjq_ctx_create(const warpjq_program *prog, uint64_t max_chunk_b
   size_t o_fs = put_table(buf, ctx->host.field_slot.data(), ctx->host.field_slot.size());
   size_t o_po = put_table(buf, prog->prefix_off, prog->n_fields);
   size_t o_pl = put_table(buf, prog->prefix_len, prog->n_fields);
+  buf.resize((buf.size() + 15) & ~(size_t)15);
+  if (buf.size() > TAB_LDS_MAX) {
+    set_err(err, err_len, "query is too large for the kernels", "its tables exceed the 16 KiB staged in LDS");
+    warpjq_ctx_destroy(ctx);
+    return WARPJQ_ERR_INVALID_ARG;
+  }
+  ctx->tab_bytes = (uint32_t)buf.size();
+  ctx->grid = main_grid(ctx);
   hipError_t e = hipMalloc(&ctx->d_tables, buf.size());
   if (e == hipSuccess) e = hipMemcpy(ctx->d_tables, buf.data(), buf.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -726,6 +734,8 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const uint8_t *src, 
     a.claim = claim;
     a.arena = ctx->arena;
     a.P = ctx->dprog;
+    a.tab = (const uint8_t *)ctx->d_tables;
+    a.tab_bytes = ctx->tab_bytes;
     a.ctr = s.d_ctr;
     a.tile_state = s.d_tile_state;
     a.parts = s.d_parts;
@@ -768,11 +778,11 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const uint8_t *src, 
       uint32_t eg = std::min<uint32_t>(ctx->grid, (uint32_t)((lines_bound + 255) / 256));
       if (eg == 0) eg = 1;
       switch (ctx->host.ns) {
-        case 1: launch_emit_1(eg, st, base, (uint32_t)nq, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
-        case 2: launch_emit_2(eg, st, base, (uint32_t)nq, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
-        case 4: launch_emit_4(eg, st, base, (uint32_t)nq, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
-        case 8: launch_emit_8(eg, st, base, (uint32_t)nq, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
-        default: launch_emit_16(eg, st, base, (uint32_t)nq, ctx->dprog, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        case 1: launch_emit_1(eg, st, base, (uint32_t)nq, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        case 2: launch_emit_2(eg, st, base, (uint32_t)nq, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        case 4: launch_emit_4(eg, st, base, (uint32_t)nq, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        case 8: launch_emit_8(eg, st, base, (uint32_t)nq, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        default: launch_emit_16(eg, st, base, (uint32_t)nq, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
       }
     }
     HIP_TRY(hipGetLastError(), "kernel chain");
