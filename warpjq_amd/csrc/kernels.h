@@ -64,11 +64,11 @@ using namespace wjqd;
 
 #define CLAIM_MIN 1024u              // smallest claim a wave takes (one scan step)
 #define CLAIM_MAX 16384u             // largest claim
-#define CLAIM_DEFAULT 4096u
+#define CLAIM_DEFAULT 8192u
 #define STEP_BYTES 1024u             // 64 lanes x 16 B
 #define TAIL_PIECE 256u              // bytes of the next claim fetched per try
 #define TAIL_MAX 1024u               // a claim's last line may end this far past it
-#define ARENA_DEFAULT 20480u         // per-wave LDS arena (bytes)
+#define ARENA_DEFAULT 24576u         // per-wave LDS arena (bytes)
 #define ARENA_MAX 32768u             // offsets fit 15 bits; bit 15 marks unstaged lines
 #define QCAP 256u                    // per-wave LDS queue of line starts
 #define QMASK (QCAP - 1u)
