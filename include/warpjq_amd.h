@@ -59,7 +59,7 @@ uint32_t warpjq_amd_kernel_width(const warpjq_ctx *ctx);
 warpjq_status warpjq_amd_set_claim(warpjq_ctx *ctx, uint32_t claim_bytes);
 
 /* LDS bytes each wave stages lines in (multiple of 256, at most 32 KiB, at
- * least the claim plus 1 KiB; default 24 KiB). Sets how many workgroups the
+ * least the claim plus 1 KiB; default 20 KiB). Sets how many workgroups the
  * one-pass kernel keeps resident. A tuning knob; results do not depend on it.
  * No counterpart in the reference. */
 warpjq_status warpjq_amd_set_arena(warpjq_ctx *ctx, uint32_t arena_bytes);

@@ -68,9 +68,9 @@ using namespace wjqd;
 #define STEP_BYTES 1024u             // 64 lanes x 16 B
 #define TAIL_PIECE 256u              // bytes of the next claim fetched per try
 #define TAIL_MAX 1024u               // a claim's last line may end this far past it
-#define ARENA_DEFAULT 24576u         // per-wave LDS arena (bytes)
+#define ARENA_DEFAULT 20480u         // per-wave LDS arena (bytes)
 #define ARENA_MAX 32768u             // offsets fit 15 bits; bit 15 marks unstaged lines
-#define QCAP 256u                    // per-wave LDS queue of line starts
+#define QCAP 128u                    // per-wave LDS queue of line starts
 #define QMASK (QCAP - 1u)
 #define Q_UNSTAGED 0x8000u
 #define LGROUP 64u                   // per-wave LDS group table entries
@@ -91,6 +91,7 @@ struct DevCtr {
   unsigned long long first_bad;  // (claim << 32 | rank) of the first invalid line, aggregate queries
   unsigned int first_invalid, overflow, saw_non_numeric, group_overflow;
   unsigned int fb_count, n_groups, ticket, tiles_done;
+  unsigned int rticket[8];  // aggregate queries: one ticket per region of the chunk
 };
 
 // Per-wave ungrouped aggregate partial.
@@ -172,32 +173,25 @@ struct DevReader {
 };
 
 // Reads a line staged in the wave's LDS arena: position i lives at arena
-// byte i + delta (delta is a multiple of 16, so blocks are aligned). Caches
-// one 16-byte block in registers. Callers only read up to the line's
-// terminator, which is staged (bytes past the chunk end are staged as '\n').
+// byte i + delta (delta is a multiple of 16, so blocks are aligned). Keeps no
+// state, so nothing it holds has to follow the walker's control flow: every
+// access is one ds_read. Callers only read up to the line's terminator, which
+// is staged (bytes past the chunk end are staged as '\n').
 typedef unsigned int wjq_v4u __attribute__((ext_vector_type(4)));
 struct LdsReader {
   const LDS_AS uint8_t *b;
   uint32_t delta;
-  uint32_t cb;
-  Blk16 v;
 
-  __device__ LdsReader(const LDS_AS uint8_t *b_, uint32_t delta_) : b(b_), delta(delta_), cb(0xFFFFFFFFu), v{0, 0, 0, 0} {}
+  __device__ LdsReader(const LDS_AS uint8_t *b_, uint32_t delta_) : b(b_), delta(delta_) {}
 
-  __device__ __forceinline__ Blk16 block(uint32_t k) {
-    if (k != cb) {
-      wjq_v4u x = *(const LDS_AS wjq_v4u *)(b + ((k << 4) + delta));
-      v = Blk16{x.x, x.y, x.z, x.w};
-      cb = k;
-    }
-    return v;
+  __device__ __forceinline__ Blk16 block(uint32_t k) const {
+    wjq_v4u x = *(const LDS_AS wjq_v4u *)(b + ((k << 4) + delta));
+    return Blk16{x.x, x.y, x.z, x.w};
   }
-  __device__ __forceinline__ uint32_t dword(uint32_t k) {
-    Blk16 x = block(k >> 2);
-    uint32_t s = k & 3;
-    return s == 0 ? x.x : s == 1 ? x.y : s == 2 ? x.z : x.w;
+  __device__ __forceinline__ uint32_t dword(uint32_t k) const {
+    return *(const LDS_AS uint32_t *)(b + ((k << 2) + delta));
   }
-  __device__ __forceinline__ uint32_t byte(uint32_t i) { return (dword(i >> 2) >> ((i & 3) * 8)) & 0xFFu; }
+  __device__ __forceinline__ uint32_t byte(uint32_t i) const { return b[i + delta]; }
 };
 
 // The program tables staged in LDS: the same fields as DProg, read with
@@ -703,84 +697,81 @@ struct Claim {
   uint32_t last_unstaged;  // start of the claim's last line when it is not wholly staged, else ~0
 };
 
-// Queues line starts of claim `c` from c.resume on, at most `room` of them;
-// when `count` is set it scans the whole claim and returns its line count.
-// Advances c.resume / c.rank; returns the number queued in *queued.
-// Wave-uniform.
+// Queues line starts of claim `c` from c.resume on, at most `room` of them,
+// and returns how many start at or after c.resume (the claim's line count on
+// the first pass). Lane l scans the contiguous slice [t0 + l*S, t0 + (l+1)*S)
+// of the staged claim (S = claim/64, 16..256 bytes), so one wave prefix sum
+// per pass orders every start. Advances c.resume / c.rank; *queued gets the
+// number queued and *last the highest start seen. Wave-uniform.
 __device__ __forceinline__ uint32_t enqueue_lines(const KArgs &a, const LDS_AS uint8_t *ar, LQueue &Q, uint32_t qtail,
-                                  uint32_t room, Claim &c, bool count, uint32_t *queued) {
+                                                  uint32_t room, Claim &c, uint32_t *queued, uint32_t *last) {
   const uint32_t lane = lane_id();
-  uint32_t k0 = (c.resume - c.t0) / STEP_BYTES;
-  uint32_t nsteps = a.claim / STEP_BYTES;
-  uint32_t carry;
-  if (k0 == 0) {
-    carry = c.carry0;
-  } else {
-    uint32_t pb = c.t0 + k0 * STEP_BYTES - 1;
-    carry = ar[pb - c.t0 + c.at] == '\n' ? 1u : 0u;
-  }
-  uint32_t seen = 0;       // starts >= c.resume seen in this pass
-  uint32_t next = c.end;   // first unqueued start
-  bool cut = false;
-  for (uint32_t k = k0; k < nsteps; k++) {
-    const uint32_t sb = c.t0 + k * STEP_BYTES;
-    if (sb >= c.end || (cut && !count)) break;
-    const uint32_t p = sb + lane * 16u;
-    uint4 v = lds_ld16(ar + (p - c.t0 + c.at));
-    uint32_t m = nl_mask16(v);
-    uint32_t up = __shfl_up(m >> 15, 1, 64) & 1u;
-    uint32_t prev = lane == 0 ? carry : up;
-    carry = __shfl(m >> 15, 63, 64) & 1u;
-    uint32_t st = ((m << 1) | prev) & 0xFFFFu;
-    // the chunk's first line starts at q = mis; nothing before it does
-    if (p + 16u <= a.mis) st = 0;
-    else if (p <= a.mis) st = (st & (0xFFFFu << (a.mis - p))) | (1u << (a.mis - p));
-    // keep starts in [c.resume, c.end)
-    if (p + 16u <= c.resume) st = 0;
-    else if (p < c.resume) st &= 0xFFFFu << (c.resume - p);
-    if (p >= c.end) st = 0;
-    else if (p + 16u > c.end) st &= (1u << (c.end - p)) - 1u;
-    uint32_t cnt = __popc(st);
-    uint32_t tot;
-    uint32_t ex = wave_excl_scan(cnt, tot);
-    if (!cut) {
-      uint32_t r = seen + ex;
-      uint32_t bits = st;
-      while (bits) {
-        uint32_t b = __builtin_ctz(bits);
-        bits &= bits - 1;
-        if (r < room) {
-          uint32_t q = p + b;
-          uint32_t e = (qtail + r) & QMASK;
-          Q.qs[e] = q;
-          Q.qi[e] = c.rank + r;
-          Q.qt[e] = c.t;
-          Q.qo[e] = (uint16_t)((q - c.t0 + c.at) | (q == c.last_unstaged ? Q_UNSTAGED : 0u));
-        } else if (r == room) {
-          next = p + b;  // exactly one lane sees rank `room`
-        }
-        r++;
-      }
-      if (seen + tot > room) {
-        cut = true;
-        // broadcast the position of the start ranked `room`
-        uint64_t own = __ballot(next != c.end);
-        uint32_t src = (uint32_t)__builtin_ctzll(own);
-        next = __shfl(next, (int)src, 64);
-      }
+  const uint32_t S = a.claim >> 6;
+  const uint32_t nb = S >> 4;  // 16-byte blocks per lane, 1..16
+  const uint32_t p0 = c.t0 + lane * S;
+  const LDS_AS uint8_t *base = ar + (c.at - c.t0);  // base[q] is the staged byte at q position q
+  uint32_t prev = lane == 0 ? c.carry0 : (base[p0 - 1] == '\n' ? 1u : 0u);
+  uint32_t st[16];
+  uint32_t cnt = 0, hi = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 16; k++) {
+    st[k] = 0;
+    if (k < nb) {
+      const uint32_t p = p0 + 16u * k;
+      const uint32_t m = nl_mask16(lds_ld16(base + p));
+      uint32_t x = ((m << 1) | prev) & 0xFFFFu;
+      prev = m >> 15;
+      // the chunk's first line starts at q = mis; nothing before it does
+      if (p + 16u <= a.mis) x = 0;
+      else if (p <= a.mis) x = (x & (0xFFFFu << (a.mis - p))) | (1u << (a.mis - p));
+      // keep starts in [c.resume, c.end)
+      if (p + 16u <= c.resume) x = 0;
+      else if (p < c.resume) x &= 0xFFFFu << (c.resume - p);
+      if (p >= c.end) x = 0;
+      else if (p + 16u > c.end) x &= (1u << (c.end - p)) - 1u;
+      st[k] = x;
+      cnt += __popc(x);
+      if (x) hi = p + 31u - (uint32_t)__builtin_clz(x);
     }
-    seen += tot;
   }
-  uint32_t q = seen < room ? seen : room;
-  *queued = q;
-  if (cut) {
+  uint32_t tot;
+  const uint32_t ex = wave_excl_scan(cnt, tot);
+  uint32_t next = c.end;  // the start ranked `room`, if the queue fills
+  uint32_t r = ex;
+#pragma unroll
+  for (uint32_t k = 0; k < 16; k++) {
+    uint32_t bits = st[k];
+    while (bits) {
+      const uint32_t bpos = (uint32_t)__builtin_ctz(bits);
+      bits &= bits - 1;
+      const uint32_t q = p0 + 16u * k + bpos;
+      if (r < room) {
+        const uint32_t e = (qtail + r) & QMASK;
+        Q.qs[e] = q;
+        Q.qi[e] = c.rank + r;
+        Q.qt[e] = c.t;
+        Q.qo[e] = (uint16_t)((q - c.t0 + c.at) | (q == c.last_unstaged ? Q_UNSTAGED : 0u));
+      } else if (r == room) {
+        next = q;
+      }
+      r++;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) hi = max(hi, (uint32_t)__shfl_xor(hi, o, 64));
+  *last = hi;
+  if (tot > room) {
+    const uint64_t own = __ballot(next != c.end);
+    next = __shfl(next, (int)__builtin_ctzll(own), 64);
     c.resume = next;
     c.rank += room;
+    *queued = room;
   } else {
     c.resume = c.end;
-    c.rank += seen;
+    c.rank += tot;
+    *queued = tot;
   }
-  return seen;
+  return tot;
 }
 
 template <int NS, bool GRP, bool LB>
@@ -826,12 +817,149 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
   Claim c{};
   const uint32_t need = a.claim + TAIL_MAX;
 
+  // Claims: row outputs take them in chunk order from one ticket (the
+  // look-back needs every earlier claim in flight); aggregate queries split
+  // the chunk into 8 regions with a ticket each, a wave starting in the
+  // region of its XCD (workgroups are dealt to the 8 XCDs round-robin) and
+  // moving on when it runs dry, which spreads the atomics and keeps a
+  // region's neighbouring claims in one XCD's L2.
+  uint32_t region = blockIdx.x & 7u, regions_tried = 0;
+  auto take_claim = [&]() -> uint32_t {
+    if (LB) {
+      uint32_t tk = 0;
+      if (lane == 0) tk = atomicAdd(&a.ctr->ticket, 1u);
+      const uint32_t t = __shfl(tk, 0, 64);
+      return t < a.ntiles ? t : ~0u;
+    }
+    while (regions_tried < 8) {
+      const uint32_t lo = (uint32_t)(((uint64_t)region * a.ntiles) >> 3);
+      const uint32_t hi = (uint32_t)(((uint64_t)(region + 1) * a.ntiles) >> 3);
+      uint32_t tk = 0;
+      if (lo < hi && lane == 0) tk = atomicAdd(&a.ctr->rticket[region], 1u);
+      tk = __shfl(tk, 0, 64);
+      if (lo + tk < hi) return lo + tk;
+      region = (region + 1) & 7u;
+      regions_tried++;
+    }
+    return ~0u;
+  };
+  // Starts the direct-to-LDS copy of claim t (and the first 256 bytes past
+  // it) to arena offset `at`; returns whether a line starts at its first
+  // byte (read through the scalar cache; waited for on use).
+  auto issue_fetch = [&](uint32_t t, uint32_t at) -> uint32_t {
+    const uint32_t t0 = t * a.claim;
+    for (uint32_t j = 0; j < a.claim + TAIL_PIECE; j += STEP_BYTES) {
+      const uint32_t x = j + lane * 16u;
+      if (x < a.claim + TAIL_PIECE && t0 + x < a.nq)
+        __builtin_amdgcn_global_load_lds((const void *)(a.d + t0 + x), (LDS_AS void *)(ar + at + j), 16, 0, 0);
+    }
+    uint32_t carry0 = 1;
+    if (t0 > 0) carry0 = (*(const uint32_t *)(a.d + t0 - 4) >> 24) == '\n' ? 1u : 0u;
+    return carry0;
+  };
+  // Keeps the bytes of queued lines at the front of the arena.
+  auto compact = [&]() {
+    if (qlen == 0) {
+      used = 0;
+      return;
+    }
+    const uint32_t lo = (uint32_t)(Q.qo[qhead & QMASK] & 0x7FFFu) & ~15u;
+    if (!lo) return;
+    for (uint32_t o = 0; o < used - lo; o += STEP_BYTES) {
+      const uint32_t x = o + lane * 16u;
+      uint4 v;
+      if (x < used - lo) v = lds_ld16(ar + lo + x);
+      __syncthreads();
+      if (x < used - lo) lds_st16(ar + x, v);
+      __syncthreads();
+    }
+    for (uint32_t e = lane; e < qlen; e += 64) {
+      const uint32_t k = (qhead + e) & QMASK;
+      Q.qo[k] = (uint16_t)(Q.qo[k] - lo);
+    }
+    used -= lo;
+    __syncthreads();
+  };
+  // Waits for claim t's bytes at `at`, stages its last line's end, and queues
+  // its line starts.
+  auto process_claim = [&](uint32_t t, uint32_t at, uint32_t carry0) {
+    const uint32_t t0 = t * a.claim;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    uint32_t span = a.claim + TAIL_PIECE;  // bytes staged for this claim
+    if (t0 + span > a.nq) {
+      for (uint32_t x = lane * 16u; x < span; x += STEP_BYTES) {
+        const uint32_t p = t0 + x;
+        if (p + 16u > a.nq) lds_st16(ar + at + x, end_patch(lds_ld16(ar + at + x), p, a.nq));
+      }
+      __syncthreads();
+    }
+    // the claim's last line must end inside the staged bytes
+    bool found = t0 + a.claim >= a.nq || ar[at + a.claim - 1] == '\n';
+    uint32_t piece = a.claim;
+    while (!found) {
+      const uint32_t x = piece + lane * 16u;
+      uint32_t m = 0;
+      if (lane < TAIL_PIECE / 16u) m = nl_mask16(lds_ld16(ar + at + x));
+      found = __ballot(m != 0) != 0;
+      piece += TAIL_PIECE;
+      if (found || piece >= a.claim + TAIL_MAX) break;
+      // one more piece
+      if (lane < TAIL_PIECE / 16u && t0 + piece + lane * 16u < a.nq)
+        __builtin_amdgcn_global_load_lds((const void *)(a.d + t0 + piece + lane * 16u),
+                                         (LDS_AS void *)(ar + at + piece), 16, 0, 0);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      span = piece + TAIL_PIECE;
+      if (t0 + span > a.nq) {
+        const uint32_t xx = piece + lane * 16u;
+        if (lane < TAIL_PIECE / 16u && t0 + xx + 16u > a.nq)
+          lds_st16(ar + at + xx, end_patch(lds_ld16(ar + at + xx), t0 + xx, a.nq));
+        __syncthreads();
+      }
+    }
+    c.t = t;
+    c.t0 = t0;
+    c.end = min(t0 + a.claim, a.nq);
+    c.at = at;
+    c.resume = t0;
+    c.rank = 0;
+    c.base = 0;
+    c.carry0 = carry0;
+    c.last_unstaged = ~0u;
+    uint32_t got, lst;
+    const uint32_t total = enqueue_lines(a, ar, Q, qhead + qlen, QCAP - qlen, c, &got, &lst);
+    if (!found && total) {
+      // the claim's last line does not end inside the staged bytes
+      c.last_unstaged = lst;
+      if (got == total) {
+        __syncthreads();
+        const uint32_t e = (qhead + qlen + got - 1) & QMASK;
+        if (lane == 0) Q.qo[e] = (uint16_t)(Q.qo[e] | Q_UNSTAGED);
+      }
+    }
+    lines_total += total;
+    if (LB) {
+      c.base = (uint32_t)tile_lookback(a.tile_state, t, total);
+      for (uint32_t e = lane; e < got; e += 64) Q.qi[(qhead + qlen + e) & QMASK] += c.base;
+    } else if (lane == 0) {
+      a.tile_state[t] = total;
+    }
+    qlen += got;
+    pending = c.resume < c.end;
+    if (total) used = at + span;
+    __syncthreads();
+  };
+
+  bool pf = false;  // a claim's copy was started before the last round
+  uint32_t pf_t = 0, pf_at = 0, pf_carry0 = 0;
+
   for (;;) {
     // ---- fill the queue to one full round when possible ----
     while (qlen < 64) {
       if (pending) {
-        uint32_t got;
-        enqueue_lines(a, ar, Q, qhead + qlen, QCAP - qlen, c, false, &got);
+        uint32_t got, lst;
+        enqueue_lines(a, ar, Q, qhead + qlen, QCAP - qlen, c, &got, &lst);
         if (LB)
           for (uint32_t e = lane; e < got; e += 64) Q.qi[(qhead + qlen + e) & QMASK] += c.base;
         qlen += got;
@@ -840,124 +968,40 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
         if (pending) break;  // the queue is full
         continue;
       }
-      if (!tiles_left) break;
-      // compact: keep the bytes of queued lines at the front of the arena
-      if (qlen == 0) {
-        used = 0;
-      } else {
-        uint32_t lo = (uint32_t)(Q.qo[qhead & QMASK] & 0x7FFFu) & ~15u;
-        if (lo) {
-          for (uint32_t o = 0; o < used - lo; o += STEP_BYTES) {
-            uint32_t x = o + lane * 16u;
-            uint4 v;
-            if (x < used - lo) v = lds_ld16(ar + lo + x);
-            __syncthreads();
-            if (x < used - lo) lds_st16(ar + x, v);
-            __syncthreads();
-          }
-          for (uint32_t e = lane; e < qlen; e += 64) {
-            uint32_t k = (qhead + e) & QMASK;
-            Q.qo[k] = (uint16_t)(Q.qo[k] - lo);
-          }
-          used -= lo;
-          __syncthreads();
-        }
+      if (pf) {
+        pf = false;
+        process_claim(pf_t, pf_at, pf_carry0);
+        continue;
       }
+      if (!tiles_left) break;
+      compact();
       if (a.arena - used < need) break;  // run a (partial) round first
-      uint32_t tk = 0;
-      if (lane == 0) tk = atomicAdd(&a.ctr->ticket, 1u);
-      const uint32_t t = __shfl(tk, 0, 64);
-      if (t >= a.ntiles) {
+      const uint32_t t = take_claim();
+      if (t == ~0u) {
         tiles_left = false;
         break;
       }
-      // ---- fetch claim t and its first tail piece into the arena ----
-      const uint32_t t0 = t * a.claim;
-      const uint32_t at = used;
-      for (uint32_t j = 0; j < a.claim + TAIL_PIECE; j += STEP_BYTES) {
-        uint32_t x = j + lane * 16u;
-        if (x < a.claim + TAIL_PIECE && t0 + x < a.nq)
-          __builtin_amdgcn_global_load_lds((const void *)(a.d + t0 + x), (LDS_AS void *)(ar + at + j), 16, 0, 0);
-      }
-      uint32_t carry0 = 1;
-      if (t0 > 0) carry0 = (*(const uint32_t *)(a.d + t0 - 4) >> 24) == '\n' ? 1u : 0u;
-      __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
-      uint32_t span = a.claim + TAIL_PIECE;  // bytes staged for this claim
-      if (t0 + span > a.nq) {
-        for (uint32_t x = lane * 16u; x < span; x += STEP_BYTES) {
-          uint32_t p = t0 + x;
-          if (p + 16u > a.nq) lds_st16(ar + at + x, end_patch(lds_ld16(ar + at + x), p, a.nq));
-        }
-        __syncthreads();
-      }
-      // the claim's last line must end inside the staged bytes
-      bool found = t0 + a.claim >= a.nq || ar[at + a.claim - 1] == '\n';
-      uint32_t piece = a.claim;
-      while (!found) {
-        uint32_t x = piece + lane * 16u;
-        uint32_t m = 0;
-        if (lane < TAIL_PIECE / 16u) m = nl_mask16(lds_ld16(ar + at + x));
-        found = __ballot(m != 0) != 0;
-        piece += TAIL_PIECE;
-        if (found || piece >= a.claim + TAIL_MAX) break;
-        // one more piece
-        if (lane < TAIL_PIECE / 16u && t0 + piece + lane * 16u < a.nq)
-          __builtin_amdgcn_global_load_lds((const void *)(a.d + t0 + piece + lane * 16u),
-                                           (LDS_AS void *)(ar + at + piece), 16, 0, 0);
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        span = piece + TAIL_PIECE;
-        if (t0 + span > a.nq) {
-          uint32_t xx = piece + lane * 16u;
-          if (lane < TAIL_PIECE / 16u && t0 + xx + 16u > a.nq)
-            lds_st16(ar + at + xx, end_patch(lds_ld16(ar + at + xx), t0 + xx, a.nq));
-          __syncthreads();
-        }
-      }
-      // ---- queue its line starts ----
-      c.t = t;
-      c.t0 = t0;
-      c.end = min(t0 + a.claim, a.nq);
-      c.at = at;
-      c.resume = t0;
-      c.rank = 0;
-      c.base = 0;
-      c.carry0 = carry0;
-      c.last_unstaged = ~0u;
-      if (!found) {
-        // the last line start of the claim: the last newline before c.end, plus one
-        uint32_t ls = 0;
-        for (uint32_t x = lane * 16u; t0 + x < c.end; x += STEP_BYTES) {
-          uint32_t m = nl_mask16(lds_ld16(ar + at + x));
-          uint32_t lim = c.end - (t0 + x);
-          if (lim < 16u) m &= (1u << lim) - 1u;
-          // a newline at the claim's last byte cannot occur here (found would be set)
-          if (m) ls = max(ls, t0 + x + 31u - (uint32_t)__builtin_clz(m) + 1u);
-        }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) ls = max(ls, (uint32_t)__shfl_xor(ls, o, 64));
-        if (t0 == 0 && ls < a.mis) ls = 0;  // newlines before the chunk start do not count
-        if (ls == 0) ls = (t0 == 0) ? a.mis : (carry0 ? t0 : ~0u);
-        c.last_unstaged = ls;
-      }
-      uint32_t got;
-      uint32_t total = enqueue_lines(a, ar, Q, qhead + qlen, QCAP - qlen, c, true, &got);
-      lines_total += total;
-      if (LB) {
-        c.base = (uint32_t)tile_lookback(a.tile_state, t, total);
-        for (uint32_t e = lane; e < got; e += 64) Q.qi[(qhead + qlen + e) & QMASK] += c.base;
-      } else if (lane == 0) {
-        a.tile_state[t] = total;
-      }
-      qlen += got;
-      pending = c.resume < c.end;
-      if (total) used = at + span;
-      __syncthreads();
+      const uint32_t carry0 = issue_fetch(t, used);
+      process_claim(t, used, carry0);
     }
     if (qlen == 0) {
-      if (!pending && !tiles_left) break;
+      if (!pending && !tiles_left && !pf) break;
       continue;
+    }
+    // aggregate queries: start copying the next claim so it lands during the round
+    if (!LB && !pf && !pending && tiles_left) {
+      compact();
+      if (a.arena - used >= need) {
+        const uint32_t t = take_claim();
+        if (t == ~0u) {
+          tiles_left = false;
+        } else {
+          pf = true;
+          pf_t = t;
+          pf_at = used;
+          pf_carry0 = issue_fetch(t, used);
+        }
+      }
     }
 
     // ---- one round: lane k evaluates queued line k ----
