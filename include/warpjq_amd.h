@@ -54,12 +54,12 @@ warpjq_status warpjq_amd_slot_timing(warpjq_ctx *ctx, uint32_t slot, float *h2d_
 uint32_t warpjq_amd_kernel_width(const warpjq_ctx *ctx);
 
 /* Bytes each wave claims per ticket in the one-pass kernel (1 KiB steps,
- * 1-16 KiB; default 8 KiB; at most the arena size minus 1 KiB). A tuning
+ * 1-8 KiB; default 8 KiB; at most the arena size minus 1 KiB). A tuning
  * knob; results do not depend on it. */
 warpjq_status warpjq_amd_set_claim(warpjq_ctx *ctx, uint32_t claim_bytes);
 
 /* LDS bytes each wave stages lines in (multiple of 256, at most 32 KiB, at
- * least the claim plus 1 KiB; default 20 KiB). Sets how many workgroups the
+ * least the claim plus 1 KiB; default 18 KiB). Sets how many workgroups the
  * one-pass kernel keeps resident. A tuning knob; results do not depend on it.
  * No counterpart in the reference. */
 warpjq_status warpjq_amd_set_arena(warpjq_ctx *ctx, uint32_t arena_bytes);

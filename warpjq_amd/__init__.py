@@ -262,7 +262,7 @@ class Context:
 
     def set_claim(self, nbytes):
         if lib().warpjq_amd_set_claim(self.ctx, nbytes) != 0:
-            raise ValueError("claim must be 1-16 KiB in 1 KiB steps and fit the arena with 1 KiB to spare")
+            raise ValueError("claim must be 1-8 KiB in 1 KiB steps and fit the arena with 1 KiB to spare")
 
     def set_arena(self, nbytes):
         if lib().warpjq_amd_set_arena(self.ctx, nbytes) != 0:

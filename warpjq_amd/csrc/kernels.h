@@ -63,16 +63,15 @@ using namespace wjqd;
 // ---------------------------------------------------------------------------
 
 #define CLAIM_MIN 1024u              // smallest claim a wave takes (one scan step)
-#define CLAIM_MAX 16384u             // largest claim
+#define CLAIM_MAX 8192u              // largest claim (line ranks fit 13 bits)
 #define CLAIM_DEFAULT 8192u
 #define STEP_BYTES 1024u             // 64 lanes x 16 B
 #define TAIL_PIECE 256u              // bytes of the next claim fetched per try
 #define TAIL_MAX 1024u               // a claim's last line may end this far past it
-#define ARENA_DEFAULT 20480u         // per-wave LDS arena (bytes)
+#define ARENA_DEFAULT 18432u         // per-wave LDS arena (bytes)
 #define ARENA_MAX 32768u             // offsets fit 15 bits; bit 15 marks unstaged lines
 #define QCAP 128u                    // per-wave LDS queue of line starts
 #define QMASK (QCAP - 1u)
-#define Q_UNSTAGED 0x8000u
 #define LGROUP 64u                   // per-wave LDS group table entries
 #define GROUP_MAX_PROBE 128u
 #define MIN_LINE_BYTES 24u           // kernels.cu:1984-1988 sizing rule
@@ -674,21 +673,29 @@ struct LGroupStore<false> {
 
 extern __shared__ uint4 wjq_dyn_lds[];
 
-// Dynamic LDS: [arena | qs | qi | qt | qo | program tables].
+#define CRING 8u  // claims with queued lines a wave tracks at once
+
+// Dynamic LDS: [arena | queue | claim ring | program tables].
 __host__ __device__ constexpr uint32_t main_lds_bytes(uint32_t arena, uint32_t tab_bytes) {
-  return arena + QCAP * (4 + 4 + 4 + 2) + ((tab_bytes + 15u) & ~15u);
+  return arena + QCAP * 4 + 128 + ((tab_bytes + 15u) & ~15u);
 }
 
-// The queue of line starts (an LDS ring of QCAP entries): q position, index
-// or rank, claim, arena offset (bit 15: the line is not wholly staged).
+// The queue of line starts: an LDS ring of QCAP entries, each
+//   bits 0-14 arena offset of the line start, bit 15 the line is not wholly
+//   staged, bits 16-18 claim-ring slot, bits 19-31 rank of the line in its
+//   claim,
+// and a ring of CRING claims with queued lines: claim index, arena offset
+// minus q position, index of the claim's first line (row outputs).
+#define QE_UNSTAGED 0x8000u
 struct LQueue {
-  LDS_AS uint32_t *qs, *qi, *qt;
-  LDS_AS uint16_t *qo;
+  LDS_AS uint32_t *qe;
+  LDS_AS uint32_t *cr_t, *cr_delta, *cr_base;
 };
 
 // The claim whose line starts are still being queued.
 struct Claim {
   uint32_t t, t0, end;  // claim index, first q position, end of its line starts
+  uint32_t slot;        // its claim-ring slot
   uint32_t at;          // arena offset of q position t0
   uint32_t resume;      // first line start not yet queued (q position)
   uint32_t rank;        // its rank in the claim
@@ -747,10 +754,8 @@ __device__ __forceinline__ uint32_t enqueue_lines(const KArgs &a, const LDS_AS u
       const uint32_t q = p0 + 16u * k + bpos;
       if (r < room) {
         const uint32_t e = (qtail + r) & QMASK;
-        Q.qs[e] = q;
-        Q.qi[e] = c.rank + r;
-        Q.qt[e] = c.t;
-        Q.qo[e] = (uint16_t)((q - c.t0 + c.at) | (q == c.last_unstaged ? Q_UNSTAGED : 0u));
+        Q.qe[e] = (q - c.t0 + c.at) | (q == c.last_unstaged ? QE_UNSTAGED : 0u) | (c.slot << 16) |
+                  ((c.rank + r) << 19);
       } else if (r == room) {
         next = q;
       }
@@ -778,10 +783,10 @@ template <int NS, bool GRP, bool LB>
 __global__ __launch_bounds__(64) void k_main(KArgs a) {
   LDS_AS uint8_t *ar = (LDS_AS uint8_t *)wjq_dyn_lds;
   LQueue Q;
-  Q.qs = (LDS_AS uint32_t *)(ar + a.arena);
-  Q.qi = Q.qs + QCAP;
-  Q.qt = Q.qi + QCAP;
-  Q.qo = (LDS_AS uint16_t *)(Q.qt + QCAP);
+  Q.qe = (LDS_AS uint32_t *)(ar + a.arena);
+  Q.cr_t = Q.qe + QCAP;
+  Q.cr_delta = Q.cr_t + CRING;
+  Q.cr_base = Q.cr_delta + CRING;
   LGroup *lgp = LGroupStore<GRP>::get();
   const LProg P = stage_prog(a.P, a.tab, a.tab_bytes, ar + main_lds_bytes(a.arena, 0));
 
@@ -816,6 +821,8 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
   bool pending = false;          // a claim still has unqueued line starts
   Claim c{};
   const uint32_t need = a.claim + TAIL_MAX;
+  uint32_t cseq = 0;  // claims given a ring slot so far
+  uint32_t hseq = 0;  // the claim of the queue's head (when the queue is not empty)
 
   // Claims: row outputs take them in chunk order from one ticket (the
   // look-back needs every earlier claim in flight); aggregate queries split
@@ -863,7 +870,7 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
       used = 0;
       return;
     }
-    const uint32_t lo = (uint32_t)(Q.qo[qhead & QMASK] & 0x7FFFu) & ~15u;
+    const uint32_t lo = (Q.qe[qhead & QMASK] & 0x7FFFu) & ~15u;
     if (!lo) return;
     for (uint32_t o = 0; o < used - lo; o += STEP_BYTES) {
       const uint32_t x = o + lane * 16u;
@@ -875,8 +882,9 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
     }
     for (uint32_t e = lane; e < qlen; e += 64) {
       const uint32_t k = (qhead + e) & QMASK;
-      Q.qo[k] = (uint16_t)(Q.qo[k] - lo);
+      Q.qe[k] -= lo;
     }
+    if (lane < CRING) Q.cr_delta[lane] -= lo;
     used -= lo;
     __syncthreads();
   };
@@ -927,6 +935,7 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
     c.base = 0;
     c.carry0 = carry0;
     c.last_unstaged = ~0u;
+    c.slot = cseq & (CRING - 1u);
     uint32_t got, lst;
     const uint32_t total = enqueue_lines(a, ar, Q, qhead + qlen, QCAP - qlen, c, &got, &lst);
     if (!found && total) {
@@ -935,15 +944,20 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
       if (got == total) {
         __syncthreads();
         const uint32_t e = (qhead + qlen + got - 1) & QMASK;
-        if (lane == 0) Q.qo[e] = (uint16_t)(Q.qo[e] | Q_UNSTAGED);
+        if (lane == 0) Q.qe[e] |= QE_UNSTAGED;
       }
     }
     lines_total += total;
-    if (LB) {
-      c.base = (uint32_t)tile_lookback(a.tile_state, t, total);
-      for (uint32_t e = lane; e < got; e += 64) Q.qi[(qhead + qlen + e) & QMASK] += c.base;
-    } else if (lane == 0) {
-      a.tile_state[t] = total;
+    if (LB) c.base = (uint32_t)tile_lookback(a.tile_state, t, total);
+    else if (lane == 0) a.tile_state[t] = total;
+    if (total) {
+      if (lane == 0) {
+        Q.cr_t[c.slot] = t;
+        Q.cr_delta[c.slot] = at - t0;
+        Q.cr_base[c.slot] = c.base;
+      }
+      if (qlen == 0) hseq = cseq;
+      cseq++;
     }
     qlen += got;
     pending = c.resume < c.end;
@@ -959,9 +973,8 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
     while (qlen < 64) {
       if (pending) {
         uint32_t got, lst;
+        if (qlen == 0) hseq = cseq - 1u;  // the pending claim is the newest
         enqueue_lines(a, ar, Q, qhead + qlen, QCAP - qlen, c, &got, &lst);
-        if (LB)
-          for (uint32_t e = lane; e < got; e += 64) Q.qi[(qhead + qlen + e) & QMASK] += c.base;
         qlen += got;
         pending = c.resume < c.end;
         __syncthreads();
@@ -974,6 +987,7 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
         continue;
       }
       if (!tiles_left) break;
+      if (qlen && cseq - hseq >= CRING) break;  // every ring slot holds queued lines
       compact();
       if (a.arena - used < need) break;  // run a (partial) round first
       const uint32_t t = take_claim();
@@ -989,7 +1003,7 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
       continue;
     }
     // aggregate queries: start copying the next claim so it lands during the round
-    if (!LB && !pf && !pending && tiles_left) {
+    if (!LB && !pf && !pending && tiles_left && cseq - hseq < CRING - 1u) {
       compact();
       if (a.arena - used >= need) {
         const uint32_t t = take_claim();
@@ -1008,12 +1022,15 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
     const uint32_t take = qlen < 64 ? qlen : 64;
     if (lane < take) {
       const uint32_t e = (qhead + lane) & QMASK;
-      const uint32_t s = Q.qs[e];
-      const uint32_t rec = Q.qi[e];
-      const uint32_t tile = Q.qt[e];
-      const uint32_t o = Q.qo[e];
-      if (!(o & Q_UNSTAGED)) {
-        LdsReader rd(ar, (o & 0x7FFFu) - s);
+      const uint32_t qv = Q.qe[e];
+      const uint32_t slot = (qv >> 16) & (CRING - 1u);
+      const uint32_t off = qv & 0x7FFFu;
+      const uint32_t s = off - Q.cr_delta[slot];
+      const uint32_t rank = qv >> 19;
+      const uint32_t rec = LB ? Q.cr_base[slot] + rank : rank;
+      const uint32_t tile = Q.cr_t[slot];
+      if (!(qv & QE_UNSTAGED)) {
+        LdsReader rd(ar, off - s);
         process_line<NS, GRP, LB>(rd, a, P, s, rec, tile, L, lgp);
       } else {
         DevReader rd(a.d, a.nq);
@@ -1022,6 +1039,11 @@ __global__ __launch_bounds__(64) void k_main(KArgs a) {
     }
     qhead = (qhead + take) & QMASK;
     qlen -= take;
+    if (qlen) {
+      // the claim of the new head: the next sequence number with its slot
+      const uint32_t hs = (Q.qe[qhead] >> 16) & (CRING - 1u);
+      hseq += (hs - hseq) & (CRING - 1u);
+    }
     __syncthreads();
   }
 
