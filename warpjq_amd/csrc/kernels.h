@@ -310,18 +310,15 @@ __device__ __forceinline__ void st_rel(unsigned long long *p, unsigned long long
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Bit 7 of each byte set when that byte is '\n' (exact per byte).
+__device__ __forceinline__ uint32_t nl_bytes(uint32_t w) {
+  const uint32_t x = w ^ 0x0A0A0A0Au;
+  return ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+}
+
 // Bit k set when byte k of the 16 is '\n' (exact per byte).
 __device__ __forceinline__ uint32_t nl_mask16(uint4 v) {
-  uint32_t w[4] = {v.x, v.y, v.z, v.w};
-  uint32_t m = 0;
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    uint32_t x = w[q] ^ 0x0A0A0A0Au;
-    uint32_t z = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
-    uint32_t bits = ((z >> 7) & 1) | ((z >> 14) & 2) | ((z >> 21) & 4) | ((z >> 28) & 8);
-    m |= bits << (4 * q);
-  }
-  return m;
+  return movemask16(nl_bytes(v.x), nl_bytes(v.y), nl_bytes(v.z), nl_bytes(v.w));
 }
 
 // The 16 bytes at q position p with every byte at or past n replaced by '\n'.
@@ -707,21 +704,21 @@ struct Claim {
 // Queues line starts of claim `c` from c.resume on, at most `room` of them,
 // and returns how many start at or after c.resume (the claim's line count on
 // the first pass). Lane l scans the contiguous slice [t0 + l*S, t0 + (l+1)*S)
-// of the staged claim (S = claim/64, 16..256 bytes), so one wave prefix sum
+// of the staged claim (S = claim/64, 16..128 bytes), so one wave prefix sum
 // per pass orders every start. Advances c.resume / c.rank; *queued gets the
 // number queued and *last the highest start seen. Wave-uniform.
 __device__ __forceinline__ uint32_t enqueue_lines(const KArgs &a, const LDS_AS uint8_t *ar, LQueue &Q, uint32_t qtail,
                                                   uint32_t room, Claim &c, uint32_t *queued, uint32_t *last) {
   const uint32_t lane = lane_id();
   const uint32_t S = a.claim >> 6;
-  const uint32_t nb = S >> 4;  // 16-byte blocks per lane, 1..16
+  const uint32_t nb = S >> 4;  // 16-byte blocks per lane, 1..8
   const uint32_t p0 = c.t0 + lane * S;
   const LDS_AS uint8_t *base = ar + (c.at - c.t0);  // base[q] is the staged byte at q position q
   uint32_t prev = lane == 0 ? c.carry0 : (base[p0 - 1] == '\n' ? 1u : 0u);
-  uint32_t st[16];
+  uint32_t st[8];
   uint32_t cnt = 0, hi = 0;
 #pragma unroll
-  for (uint32_t k = 0; k < 16; k++) {
+  for (uint32_t k = 0; k < 8; k++) {
     st[k] = 0;
     if (k < nb) {
       const uint32_t p = p0 + 16u * k;
@@ -746,7 +743,7 @@ __device__ __forceinline__ uint32_t enqueue_lines(const KArgs &a, const LDS_AS u
   uint32_t next = c.end;  // the start ranked `room`, if the queue fills
   uint32_t r = ex;
 #pragma unroll
-  for (uint32_t k = 0; k < 16; k++) {
+  for (uint32_t k = 0; k < 8; k++) {
     uint32_t bits = st[k];
     while (bits) {
       const uint32_t bpos = (uint32_t)__builtin_ctz(bits);
@@ -779,8 +776,10 @@ __device__ __forceinline__ uint32_t enqueue_lines(const KArgs &a, const LDS_AS u
   return tot;
 }
 
+// Two waves per SIMD is what the LDS arena allows; the register budget is
+// held to that so the allocator never spills into AGPRs and drops to one.
 template <int NS, bool GRP, bool LB>
-__global__ __launch_bounds__(64) void k_main(KArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_main(KArgs a) {
   LDS_AS uint8_t *ar = (LDS_AS uint8_t *)wjq_dyn_lds;
   LQueue Q;
   Q.qe = (LDS_AS uint32_t *)(ar + a.arena);
