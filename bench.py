@@ -8,6 +8,12 @@ three queries of the same workload (kernel-only GB/s), the end-to-end
 host-buffer run (pinned H2D + kernels), the roofline of the main kernel
 (hipEvents around it, on its own stream) and the CPU baseline (the oracle on
 a bounded sample, host threads).
+
+`--workload files` measures the batched form instead: 4,096 files of
+lognormal sizes clamped to [4 KB, 256 MB], 16 GiB in all, 1% of them holding
+a line of up to 1 MB, resident in HBM; the filter-count over all of them
+once per file (one launch each, two slots in flight) and batched (spans of
+many files per launch, warpjq_amd_submit_spans).
 """
 import argparse
 import json
@@ -40,6 +46,139 @@ def split_points(host, n, piece):
     return list(zip(pts[:-1], pts[1:]))
 
 
+def file_sizes(rng, n_files, total, lo=4000, hi=256_000_000):
+    """Lognormal file sizes clamped to [lo, hi], scaled to sum to `total`."""
+    import numpy as np
+    s = rng.lognormal(mean=np.log(1 << 20), sigma=2.0, size=n_files)
+    for _ in range(50):
+        s = np.clip(s * (total / s.sum()), lo, hi)
+    return s.astype(np.int64)
+
+
+def bench_files(args, w, torch, np):
+    n_files = args.files
+    total = int(args.size_gib * (1 << 30)) if args.size_gib != 8.0 else 16 << 30
+    rng = np.random.default_rng(args.seed)
+    t0 = time.time()
+    host, n, n_lines, hist = w.gen_nginx(total, seed=args.seed, threads=min(16, os.cpu_count() or 1))
+    log(f"generated {n} bytes, {n_lines} lines in {time.time() - t0:.1f}s")
+    sizes = file_sizes(rng, n_files, n)
+    # newline-aligned file boundaries near the drawn sizes
+    bounds, a = [], 0
+    for k, sz in enumerate(sizes):
+        if a >= n:
+            break
+        b = n if k == n_files - 1 else min(n, a + int(sz))
+        while b < n and host[b - 1] != 10:
+            b += 1
+        bounds.append((a, b))
+        a = b
+    if a < n:
+        bounds[-1] = (bounds[-1][0], n)
+    want = hist[500]
+    # 1% of the files get one long line (64 KB - 1 MB) in place of the lines it covers
+    n_long = int(round(len(bounds) * args.long_frac))
+    long_files = rng.choice(len(bounds), size=n_long, replace=False) if n_long else []
+    for f in long_files:
+        a, b = bounds[f]
+        ln = int(min(b - a, rng.integers(64 << 10, 1 << 20)))
+        s0 = a
+        e0 = s0 + ln
+        while e0 < b and host[e0 - 1] != 10:
+            e0 += 1
+        if e0 - s0 < 128:
+            continue
+        region = host[s0:e0].tobytes()
+        want -= region.count(b'"status":500,')
+        head = b'{"ts":1,"host":"h","method":"GET","path":"/long","status":200,"bytes":1,"ua":"'
+        body = head + b"x" * (e0 - s0 - len(head) - 3) + b'"}\n'
+        host[s0:e0] = np.frombuffer(body, dtype=np.uint8)
+    d = torch.from_numpy(host).to("cuda")
+    torch.cuda.synchronize()
+    base = d.data_ptr()
+    ctx = w.Context(QUERY, chunk_bytes=64 << 20, n_slots=2)
+
+    def per_file():
+        count, pend = 0, None
+        for i, (a, b) in enumerate(bounds):
+            ctx.submit_device(i & 1, base + a, b - a)
+            if pend is not None:
+                count += ctx.wait(pend).agg.count
+            pend = i & 1
+        count += ctx.wait(pend).agg.count
+        return count
+
+    # batched: as many files per launch as fit 4 GiB of positions (65,536 spans)
+    groups, cur, pos = [], [], 0
+    claim = 8192
+    for a, b in bounds:
+        v = ((b - a) + 15 + claim - 1) // claim * claim
+        if cur and (pos + v > (4 << 30) - (2 << 20) or len(cur) == 65536):
+            groups.append(cur)
+            cur, pos = [], 0
+        cur.append((base + a, b - a))
+        pos += v
+    if cur:
+        groups.append(cur)
+
+    launch_ms = []
+
+    def batched():
+        count, pend = 0, None
+        launch_ms.clear()
+        for i, g in enumerate(groups):
+            ctx.submit_spans(i & 1, g)
+            if pend is not None:
+                count += ctx.wait(pend).agg.count
+                launch_ms.append(round(ctx.timing(pend)[2], 3))
+            pend = i & 1
+        count += ctx.wait(pend).agg.count
+        launch_ms.append(round(ctx.timing(pend)[2], 3))
+        return count
+
+    res = {}
+    for name, fn in (("per_file", per_file), ("batched", batched)):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        best = None
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            c = fn()
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+            if c != want:
+                raise SystemExit(f"parity: {name} count {c} != generator count {want}")
+        res[name] = {"ms": round(best * 1e3, 2), "gbps": round(n / best / 1e9, 1)}
+        log(name, res[name])
+    speedup = res["per_file"]["ms"] / res["batched"]["ms"]
+    line = {
+        "metric": "input GB/s of select(.status == 500) | count over a 4,096-file NDJSON mix resident in HBM "
+                  "(batched form)",
+        "value": res["batched"]["gbps"],
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["batched"]["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": f"{len(bounds)} nginx NDJSON files, lognormal sizes clamped to [4 KB, 256 MB], "
+                               f"{len(long_files)} with a 64 KB-1 MB line, resident in HBM",
+                   "bytes": n, "files": len(bounds), "launches_batched": len(groups),
+                   "smallest": int(min(b - a for a, b in bounds)), "largest": int(max(b - a for a, b in bounds)),
+                   "grid": ctx.grid},
+        "per_file": res["per_file"],
+        "batched": res["batched"],
+        "speedup_batched_vs_per_file": round(speedup, 2),
+        "batched_main_ms_per_launch": launch_ms,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -48,6 +187,9 @@ def main():
     ap.add_argument("--size-gib", type=float, default=8.0)
     ap.add_argument("--piece-mib", type=int, default=4095)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--workload", choices=["nginx", "files"], default="nginx")
+    ap.add_argument("--files", type=int, default=4096)
+    ap.add_argument("--long-frac", type=float, default=0.01, help="files mix: share of files with a long line")
     ap.add_argument("--no-extras", action="store_true", help="only the timed filter-count loop")
     ap.add_argument("--cpu-sample-mib", type=int, default=1024)
     ap.add_argument("--claim-kib", type=int, default=0, help="bytes per wave claim (tuning)")
@@ -67,6 +209,8 @@ def main():
 
     w.probe()
     dev = w.device_name()
+    if args.workload == "files":
+        return bench_files(args, w, torch, np)
     target = int(args.size_gib * (1 << 30))
     t0 = time.time()
     host, n, n_lines, hist = w.gen_nginx(target, seed=args.seed, threads=min(16, os.cpu_count() or 1))

@@ -44,11 +44,66 @@ warpjq_status warpjq_amd_wait_ext(warpjq_ctx *ctx, uint32_t slot, warpjq_amd_agg
 warpjq_status warpjq_amd_submit_device(warpjq_ctx *ctx, uint32_t slot, const uint8_t *d_bytes,
                                        uint64_t n_bytes, char *err, size_t err_len);
 
+/* One span of a batched submit: bytes already in device memory, newline
+ * complete (a file, or a newline-aligned piece of one), with at least 256
+ * readable bytes past n_bytes. */
+typedef struct {
+  const uint8_t *d_bytes;
+  uint64_t n_bytes;
+} warpjq_amd_span;
+
+/* The batched form: every span of `spans` in ONE persistent launch whose
+ * waves take claims from any span (work stealing), as one stream in span
+ * order, the way the reference chains several inputs
+ * (crates/warpjq-core/src/chunk.rs:46-57, :198-203): chunks never cross a
+ * span, and a span's last line ends at its end. Results are those of one
+ * chunk holding the spans in order (line indices continue across spans).
+ * At most 65,536 spans whose sizes, each rounded up to the claim size, sum to
+ * at most 4 GiB - 1 MiB; row outputs also within the slot capacity. After
+ * warpjq_wait, fallback offsets are relative to their span
+ * (warpjq_amd_fallback_spans gives the span of each), and groups[].key_off
+ * indexes warpjq_amd_group_keys (16 MiB of key bytes per submit; more sets
+ * group_overflow). No counterpart in the reference. */
+warpjq_status warpjq_amd_submit_spans(warpjq_ctx *ctx, uint32_t slot, const warpjq_amd_span *spans,
+                                      uint32_t n_spans, char *err, size_t err_len);
+/* The batched form over the slot's pinned buffer: copies its first n_bytes
+ * to the device (as warpjq_submit does) and runs every piece (off, len
+ * within those bytes; newline complete) in one launch, with the semantics of
+ * warpjq_amd_submit_spans. For a host that packs many small files into one
+ * slot. No counterpart in the reference. */
+typedef struct {
+  uint64_t off;
+  uint64_t len;
+} warpjq_amd_piece;
+warpjq_status warpjq_amd_submit_pieces(warpjq_ctx *ctx, uint32_t slot, uint64_t n_bytes,
+                                       const warpjq_amd_piece *pieces, uint32_t n_pieces, char *err,
+                                       size_t err_len);
+
+/* Copies n host bytes into a new device allocation followed by 256 newline
+ * bytes (so it can be submitted resident), synchronously; NULL on failure.
+ * Free with warpjq_amd_device_free. No counterpart in the reference. */
+uint8_t *warpjq_amd_device_upload(const uint8_t *bytes, uint64_t n, char *err, size_t err_len);
+void warpjq_amd_device_free(uint8_t *p);
+
+/* After warpjq_wait on a batched submit: the span (piece) index of each
+ * fallback line, parallel to fallback_idx; and the key bytes that
+ * groups[].key_off indexes. Valid until the next submit on the slot. No
+ * counterpart in the reference. */
+const uint32_t *warpjq_amd_fallback_spans(const warpjq_ctx *ctx, uint32_t slot);
+const uint8_t *warpjq_amd_group_keys(const warpjq_ctx *ctx, uint32_t slot);
+
 /* Device time of the last submit on `slot` (hipEvent on the slot's stream):
  * the H2D copy, the whole kernel chain, and the one-pass main kernel alone.
  * Valid after warpjq_wait. No counterpart in the reference. */
 warpjq_status warpjq_amd_slot_timing(warpjq_ctx *ctx, uint32_t slot, float *h2d_ms, float *kernel_ms,
                                      float *main_ms);
+
+/* Points an idle context (no slot in flight) at another program whose
+ * output needs the same buffers (both grouped, both ungrouped aggregates, or
+ * both row outputs): uploads its tables, keeps every allocation. Lets a
+ * server reuse contexts across queries that differ only in constants. On
+ * error the old program stays. No counterpart in the reference. */
+warpjq_status warpjq_amd_ctx_reprogram(warpjq_ctx *ctx, const warpjq_program *prog, char *err, size_t err_len);
 
 /* Slot-table width of the kernel instantiation a context uses (1..16). */
 uint32_t warpjq_amd_kernel_width(const warpjq_ctx *ctx);
@@ -90,7 +145,46 @@ int warpjq_amd_run_buffer(const char *query, const uint8_t *data, uint64_t n, in
                           uint64_t chunk_bytes, uint64_t max_line_bytes, uint8_t **out, uint64_t *out_len,
                           warpjq_amd_stats *stats, char *err, size_t err_len);
 
+/* The same over several inputs read as one stream, as the reference reads a
+ * list of files (Input::Chained, chunk.rs:46-57, :198-203): aggregates total
+ * across files, a chunk never crosses a file, each file's BOM is dropped and
+ * its last line is complete at its end, line numbers continue. Small files
+ * are packed into one slot and run as one launch (warpjq_amd_submit_pieces).
+ * Same arguments and return codes as warpjq_amd_run_buffer. */
+int warpjq_amd_run_files(const char *query, const uint8_t *const *files, const uint64_t *lens, uint32_t n_files,
+                         int format, int on_invalid, uint64_t chunk_bytes, uint64_t max_line_bytes, uint8_t **out,
+                         uint64_t *out_len, warpjq_amd_stats *stats, char *err, size_t err_len);
+
 void warpjq_amd_free(void *p);
+
+/* The served form. A server keeps datasets resident in HBM within
+ * `hbm_budget` bytes (loaded on first use, least recently used evicted when
+ * no running request holds them) and runs queued requests on `workers`
+ * threads, each with its own device contexts and streams: requests overlap on
+ * the GPU and a worker takes the next admissible request as soon as it
+ * finishes one. A request's result is exactly warpjq_amd_run_files' over its
+ * datasets in order. No counterpart in the reference. */
+typedef struct warpjq_amd_server warpjq_amd_server;
+typedef struct {
+  uint64_t budget, reserved, loads, evictions, completed, resident;
+  double p50_ms, p99_ms, max_ms; /* arrival to result, completed requests */
+} warpjq_amd_server_stats;
+
+warpjq_amd_server *warpjq_amd_server_create(uint64_t hbm_budget, uint32_t workers, uint64_t chunk_bytes, char *err,
+                                            size_t err_len);
+void warpjq_amd_server_destroy(warpjq_amd_server *s);
+/* Registers host bytes that stay valid and unchanged for the server's life
+ * (they back row chunking and overflow splits). Returns the dataset id. */
+int64_t warpjq_amd_server_add_dataset(warpjq_amd_server *s, const uint8_t *bytes, uint64_t n);
+/* Queues `query` over the datasets, in order, read as one stream. Returns
+ * the request id, or -1 on a bad argument. */
+int64_t warpjq_amd_server_submit(warpjq_amd_server *s, const char *query, const uint32_t *datasets,
+                                 uint32_t n_datasets, int format, int on_invalid);
+/* Blocks until request `id` is done; then as warpjq_amd_run_buffer returns,
+ * plus its latency from submit to result. */
+int warpjq_amd_server_wait(warpjq_amd_server *s, int64_t id, uint8_t **out, uint64_t *out_len,
+                           warpjq_amd_stats *stats, double *latency_ms, char *err, size_t err_len);
+void warpjq_amd_server_get_stats(warpjq_amd_server *s, warpjq_amd_server_stats *out);
 
 /* Query text -> lowered program (the reference builds it in Rust:
  * query/parse.rs:59-75, query/mod.rs:95-131, gpu/lower.rs:38-160). The

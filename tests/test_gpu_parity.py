@@ -87,3 +87,35 @@ def test_nginx_filter_count():
         cpu, cst, gpu, gst = both(q, data, chunk=1 << 20)
         assert gpu == cpu, q
         assert gst["lines_in"] == cst["lines_in"]
+
+
+def long_lines_corpus(seed):
+    """Short lines around lines of 9 KB - 1.2 MB: long strings, long arrays,
+    deep objects, a long invalid line and a long line whose key is late."""
+    import random
+    rng = random.Random(seed)
+    short = corpus.nginx(seed, 400).split(b"\n")[:-1]
+    longs = []
+    for n in (9000, 20000, 70000, 300000, 1200000):
+        longs.append(b'{"status":500,"bytes":%d,"host":"h-long","ua":"%s"}' % (n, b"x" * n))
+        longs.append(b'{"ua":"%s","status":404,"host":"late","bytes":7}' % (b"y\\\"z" * (n // 4)))
+        longs.append(b'{"arr":[%s],"status":200,"bytes":1,"host":"arr"}' % b",".join(b"%d" % i for i in range(n // 8)))
+        longs.append(b'{"status":500,"host":"bad","ua":"%s' % (b"w" * n))  # unterminated string: invalid
+    deep = b'{"n":' * 50 + b"1" + b"}" * 50
+    longs.append(b'{"status":500,"host":"deep","pad":"%s","d":%s}' % (b"p" * 20000, deep))
+    lines = short[:]
+    for ln in longs:
+        lines.insert(rng.randrange(len(lines) + 1), ln)
+    return b"\n".join(lines) + b"\n"
+
+
+def test_long_lines():
+    data = long_lines_corpus(4)
+    for q in ("count", "select(.status == 500) | count", "sum(.bytes)", "max(.bytes)",
+              "group_by(.host) | count", "group_by(.host) | sum(.bytes)", "select(.status == 404) | {h: .host}",
+              ".host", "select(.bytes > 100000) | .bytes"):
+        for fmt in (w.FMT_NDJSON, w.FMT_COUNT):
+            cpu, cst, gpu, gst = both(q, data, fmt=fmt, chunk=1 << 22)
+            assert gpu == cpu, (q, fmt)
+            for k in ("lines_in", "malformed", "type_errors"):
+                assert gst[k] == cst[k], (q, k)

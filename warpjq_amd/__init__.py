@@ -79,6 +79,15 @@ class AggExt(ctypes.Structure):
     _fields_ = [("isum_lo", u64), ("isum_hi", u64), ("fsum", f64), ("fcomp", f64)]
 
 
+class Span(ctypes.Structure):
+    _fields_ = [("d_bytes", ctypes.c_void_p), ("n_bytes", u64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("budget", u64), ("reserved", u64), ("loads", u64), ("evictions", u64), ("completed", u64),
+                ("resident", u64), ("p50_ms", f64), ("p99_ms", f64), ("max_ms", f64)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("bytes_in", u64), ("lines_in", u64), ("lines_out", u64), ("malformed", u64),
                 ("type_errors", u64), ("gpu_fallback_lines", u64), ("gpu_redone_chunks", u64),
@@ -91,7 +100,12 @@ REFERENCE_SYMBOLS = ["warpjq_probe", "warpjq_abi_check", "warpjq_ctx_create", "w
                      "warpjq_wait", "warpjq_device_name"]
 ADDITIVE_SYMBOLS = ["warpjq_amd_ext_value", "warpjq_amd_wait_ext", "warpjq_amd_submit_device",
                     "warpjq_amd_slot_timing", "warpjq_amd_kernel_width", "warpjq_amd_set_claim", "warpjq_amd_set_arena",
-                    "warpjq_amd_grid", "warpjq_amd_run_buffer",
+                    "warpjq_amd_grid", "warpjq_amd_submit_spans", "warpjq_amd_fallback_spans",
+                    "warpjq_amd_group_keys", "warpjq_amd_submit_pieces", "warpjq_amd_run_buffer",
+                    "warpjq_amd_run_files", "warpjq_amd_device_upload", "warpjq_amd_device_free",
+                    "warpjq_amd_server_create", "warpjq_amd_server_destroy", "warpjq_amd_server_add_dataset",
+                    "warpjq_amd_server_submit", "warpjq_amd_server_wait", "warpjq_amd_server_get_stats",
+                    "warpjq_amd_ctx_reprogram",
                     "warpjq_amd_free", "warpjq_amd_compile", "warpjq_amd_program_view",
                     "warpjq_amd_program_free", "warpjq_amd_gen_nginx"]
 
@@ -136,10 +150,37 @@ def lib():
         L.warpjq_amd_set_claim.argtypes = [ctypes.c_void_p, u32]
         L.warpjq_amd_set_arena.argtypes = [ctypes.c_void_p, u32]
         L.warpjq_amd_grid.argtypes = [ctypes.c_void_p]
+        L.warpjq_amd_submit_spans.argtypes = [ctypes.c_void_p, u32, P(Span), u32, cp, sz]
+        L.warpjq_amd_fallback_spans.argtypes = [ctypes.c_void_p, u32]
+        L.warpjq_amd_fallback_spans.restype = P(u32)
+        L.warpjq_amd_group_keys.argtypes = [ctypes.c_void_p, u32]
+        L.warpjq_amd_group_keys.restype = P(ctypes.c_uint8)
         L.warpjq_amd_grid.restype = u32
         L.warpjq_amd_run_buffer.argtypes = [cp, ctypes.c_void_p, u64, ctypes.c_int, ctypes.c_int, u64, u64,
                                             P(P(ctypes.c_uint8)), P(u64), P(Stats), cp, sz]
         L.warpjq_amd_run_buffer.restype = ctypes.c_int
+        L.warpjq_amd_run_files.argtypes = [cp, P(ctypes.c_void_p), P(u64), u32, ctypes.c_int, ctypes.c_int, u64,
+                                           u64, P(P(ctypes.c_uint8)), P(u64), P(Stats), cp, sz]
+        L.warpjq_amd_run_files.restype = ctypes.c_int
+        L.warpjq_amd_submit_pieces.argtypes = [ctypes.c_void_p, u32, u64, ctypes.c_void_p, u32, cp, sz]
+        L.warpjq_amd_ctx_reprogram.argtypes = [ctypes.c_void_p, P(Program), cp, sz]
+        L.warpjq_amd_device_upload.argtypes = [ctypes.c_void_p, u64, cp, sz]
+        L.warpjq_amd_device_upload.restype = ctypes.c_void_p
+        L.warpjq_amd_device_free.argtypes = [ctypes.c_void_p]
+        L.warpjq_amd_device_free.restype = None
+        L.warpjq_amd_server_create.argtypes = [u64, u32, u64, cp, sz]
+        L.warpjq_amd_server_create.restype = ctypes.c_void_p
+        L.warpjq_amd_server_destroy.argtypes = [ctypes.c_void_p]
+        L.warpjq_amd_server_destroy.restype = None
+        L.warpjq_amd_server_add_dataset.argtypes = [ctypes.c_void_p, ctypes.c_void_p, u64]
+        L.warpjq_amd_server_add_dataset.restype = ctypes.c_int64
+        L.warpjq_amd_server_submit.argtypes = [ctypes.c_void_p, cp, P(u32), u32, ctypes.c_int, ctypes.c_int]
+        L.warpjq_amd_server_submit.restype = ctypes.c_int64
+        L.warpjq_amd_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_int64, P(P(ctypes.c_uint8)), P(u64),
+                                             P(Stats), P(f64), cp, sz]
+        L.warpjq_amd_server_wait.restype = ctypes.c_int
+        L.warpjq_amd_server_get_stats.argtypes = [ctypes.c_void_p, P(ServerStats)]
+        L.warpjq_amd_server_get_stats.restype = None
         L.warpjq_amd_free.argtypes = [ctypes.c_void_p]
         L.warpjq_amd_free.restype = None
         L.warpjq_amd_compile.argtypes = [cp, ctypes.c_int, cp, sz]
@@ -202,6 +243,27 @@ def _addr(data, n=None):
     return ctypes.addressof(buf), len(data) if n is None else n
 
 
+def run_files(query, files, fmt=FMT_NDJSON, on_invalid=SKIP, chunk_bytes=GPU_CHUNK, max_line=MAX_LINE):
+    """Runs `query` over a list of inputs read as one stream (the reference's
+    multi-file form) on the GPU. Returns (output bytes, stats)."""
+    L = lib()
+    keep = list(files)
+    n = len(keep)
+    ptrs = (ctypes.c_void_p * max(1, n))()
+    lens = (u64 * max(1, n))()
+    for i, f in enumerate(keep):
+        ptrs[i], lens[i] = _addr(f)
+    out = P(ctypes.c_uint8)()
+    out_len = u64()
+    st = Stats()
+    e = ctypes.create_string_buffer(2048)
+    q = query.encode() if isinstance(query, str) else query
+    rc = L.warpjq_amd_run_files(q, ptrs, lens, n, fmt, on_invalid, chunk_bytes, max_line, ctypes.byref(out),
+                                ctypes.byref(out_len), ctypes.byref(st), e, 2048)
+    del keep
+    return _finish(L, rc, out, out_len, st, e)
+
+
 def run(query, data, fmt=FMT_NDJSON, on_invalid=SKIP, chunk_bytes=GPU_CHUNK, max_line=MAX_LINE, n=None):
     """Runs `query` over `data` (bytes, or a numpy uint8 array whose first
     `n` bytes are the input) on the GPU. Returns (output bytes, stats)."""
@@ -216,6 +278,10 @@ def run(query, data, fmt=FMT_NDJSON, on_invalid=SKIP, chunk_bytes=GPU_CHUNK, max
     rc = L.warpjq_amd_run_buffer(q, addr, nbytes, fmt, on_invalid, chunk_bytes, max_line, ctypes.byref(out),
                                  ctypes.byref(out_len), ctypes.byref(st), e, 2048)
     del keep
+    return _finish(L, rc, out, out_len, st, e)
+
+
+def _finish(L, rc, out, out_len, st, e):
     stats = {k: getattr(st, k) for k, _ in Stats._fields_}
     if rc != 0:
         raise RunError(rc, st.err_line, e.value.decode(errors="replace"))
@@ -260,6 +326,15 @@ class Context:
     def __del__(self):
         self.close()
 
+    def reprogram(self, query, csv=False):
+        """Points this (idle) context at another query with the same output shape."""
+        prog = Compiled(query, csv)
+        e = _err()
+        rc = lib().warpjq_amd_ctx_reprogram(self.ctx, prog.view, e, 512)
+        if rc != 0:
+            raise DeviceError(f"reprogram: {STATUS_NAMES.get(rc, rc)}: {e.value.decode()}")
+        self.prog = prog
+
     def set_claim(self, nbytes):
         if lib().warpjq_amd_set_claim(self.ctx, nbytes) != 0:
             raise ValueError("claim must be 1-8 KiB in 1 KiB steps and fit the arena with 1 KiB to spare")
@@ -296,6 +371,26 @@ class Context:
         if rc != 0:
             raise DeviceError(f"submit_device: {STATUS_NAMES.get(rc, rc)}: {e.value.decode()}")
 
+    def submit_spans(self, slot, spans):
+        """One launch over device spans [(dptr, n_bytes), ...] (the batched form)."""
+        arr = (Span * max(1, len(spans)))()
+        for i, (p, n) in enumerate(spans):
+            arr[i].d_bytes = p
+            arr[i].n_bytes = n
+        e = _err()
+        rc = lib().warpjq_amd_submit_spans(self.ctx, slot, arr, len(spans), e, 512)
+        if rc != 0:
+            raise DeviceError(f"submit_spans: {STATUS_NAMES.get(rc, rc)}: {e.value.decode()}")
+
+    def fallback_spans(self, slot, n):
+        """Span index of each of the `n` fallback lines of the last wait on `slot`."""
+        p = lib().warpjq_amd_fallback_spans(self.ctx, slot)
+        return [p[i] for i in range(n)]
+
+    def group_key(self, slot, off, n):
+        """Key bytes of a group of a multi-span submit (key_off indexes the key blob)."""
+        return ctypes.string_at(ctypes.addressof(lib().warpjq_amd_group_keys(self.ctx, slot).contents) + off, n)
+
     def wait(self, slot):
         r = ChunkResult()
         e = _err()
@@ -315,6 +410,81 @@ class Context:
         a, b, m = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
         lib().warpjq_amd_slot_timing(self.ctx, slot, ctypes.byref(a), ctypes.byref(b), ctypes.byref(m))
         return a.value, b.value, m.value
+
+
+class DeviceBuffer:
+    """Host bytes copied to a new device allocation (+256 newline bytes)."""
+
+    def __init__(self, data):
+        addr, n = _addr(data)
+        e = _err()
+        self.ptr = lib().warpjq_amd_device_upload(addr, n, e, 512)
+        if not self.ptr:
+            raise DeviceError(f"device_upload: {e.value.decode()}")
+        self.n = n
+
+    def free(self):
+        if self.ptr:
+            lib().warpjq_amd_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+class Server:
+    """The served form (warpjq_amd_server_*): resident datasets under an HBM
+    budget, requests run by `workers` threads as they are admitted."""
+
+    def __init__(self, hbm_budget, workers=4, chunk_bytes=GPU_CHUNK):
+        e = _err()
+        self.h = lib().warpjq_amd_server_create(hbm_budget, workers, chunk_bytes, e, 512)
+        if not self.h:
+            raise DeviceError(e.value.decode())
+        self._keep = []
+
+    def add_dataset(self, data):
+        """Registers bytes (kept referenced for the server's life)."""
+        self._keep.append(data)
+        addr, n = _addr(data)
+        i = lib().warpjq_amd_server_add_dataset(self.h, addr, n)
+        if i < 0:
+            raise ValueError("add_dataset failed")
+        return i
+
+    def submit(self, query, datasets, fmt=FMT_NDJSON, on_invalid=SKIP):
+        ids = (u32 * max(1, len(datasets)))(*datasets)
+        q = query.encode() if isinstance(query, str) else query
+        r = lib().warpjq_amd_server_submit(self.h, q, ids, len(datasets), fmt, on_invalid)
+        if r < 0:
+            raise ValueError("server_submit: bad argument")
+        return r
+
+    def wait(self, req):
+        """(output bytes, stats, latency_ms); raises RunError like run()."""
+        L = lib()
+        out = P(ctypes.c_uint8)()
+        out_len = u64()
+        st = Stats()
+        lat = f64()
+        e = ctypes.create_string_buffer(2048)
+        rc = L.warpjq_amd_server_wait(self.h, req, ctypes.byref(out), ctypes.byref(out_len), ctypes.byref(st),
+                                      ctypes.byref(lat), e, 2048)
+        b, stats = _finish(L, rc, out, out_len, st, e)
+        return b, stats, lat.value
+
+    def stats(self):
+        s = ServerStats()
+        lib().warpjq_amd_server_get_stats(self.h, ctypes.byref(s))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().warpjq_amd_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 STATUS_ORDER = (200, 404, 500, 301, 304, 403, 502, 503)

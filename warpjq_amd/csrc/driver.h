@@ -1,0 +1,71 @@
+// Internal interface of the host driver (host.cpp), shared with the
+// serving loop (serve.cpp). Not exported.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/warpjq_amd.h"
+#include "../../include/warpjq_kernels.h"
+#include "query.h"
+
+namespace wjqd {
+
+enum Format { F_NDJSON = 0, F_CSV = 1, F_COUNT = 2 };
+enum OnInvalid { OI_WARN = 0, OI_SKIP = 1, OI_ABORT = 2 };
+
+struct Chunk {
+  uint64_t off, len;
+};
+
+// One submit's input: newline-complete pieces in stream order (a chunk of
+// one buffer, or several small files packed into one slot).
+struct Piece {
+  const uint8_t *p;  // host bytes (always: overflow splits and row cuts need them)
+  uint64_t len;
+  const uint8_t *d = nullptr;  // the same bytes resident in device memory, or null
+};
+using Batch = std::vector<Piece>;
+
+// Batches over a list of inputs read as one stream, the way the reference
+// chains files (chunk.rs:46-57, :198-203): a chunk never crosses an input,
+// each input's BOM is dropped (chunk.rs:210-212) and its last line is
+// complete at its end. Whole small inputs share a batch; a large one is cut
+// into newline-aligned chunks (gpu/mod.rs:697-810).
+struct Batcher {
+  const uint8_t *const *files;
+  const uint64_t *lens;
+  uint32_t n_files;
+  uint64_t cap, max_lines, max_line;
+  uint32_t max_pieces;
+  const uint8_t *const *dfiles = nullptr;  // device copies of `files`, or null
+  uint32_t fi = 0;
+  uint64_t pos = 0;
+  bool started = false;
+
+  bool next(Batch &b, std::string &err);
+};
+
+// A lowered query with its device context, reusable run after run (one
+// thread at a time).
+struct Engine {
+  wjq::Program prog;
+  wjq::Lowered low;
+  warpjq_program view{};
+  warpjq_ctx *ctx = nullptr;
+  uint64_t cap = 0, max_lines = 0;
+  bool csv = false;
+};
+
+// 0 ok, 1 query error, 3 other error (message in err).
+int engine_open(Engine &g, const char *query, int format, uint64_t chunk_bytes, char *err, size_t err_len);
+// Points an open engine at another query whose output needs the same device
+// buffers, keeping its context: 0 ok, 1 query error, 2 not compatible (open
+// another engine), 3 other error.
+int engine_reprogram(Engine &g, const char *query, int format, char *err, size_t err_len);
+void engine_close(Engine &g);
+// Runs every batch `src` yields; the return codes of warpjq_amd_run_buffer.
+int engine_run(Engine &g, Batcher &src, int format, int on_invalid, uint8_t **out, uint64_t *out_len,
+               warpjq_amd_stats *stats, char *err, size_t err_len);
+
+}  // namespace wjqd

@@ -30,12 +30,11 @@
 
 #include "../../include/warpjq_amd.h"
 #include "../../include/warpjq_kernels.h"
+#include "driver.h"
 #include "query.h"
 
-namespace {
+namespace wjqd {
 
-enum Format { F_NDJSON = 0, F_CSV = 1, F_COUNT = 2 };
-enum OnInvalid { OI_WARN = 0, OI_SKIP = 1, OI_ABORT = 2 };
 
 // ---- aggregates (agg.rs:15-98) with exact sum parts ----------------------
 
@@ -269,14 +268,16 @@ static std::string group_key_name(const wjq::Program &p) {
   return steps.back().key;
 }
 
-struct Chunk {
-  uint64_t off, len;
-};
+
+
+static uint64_t batch_bytes(const Batch &b) {
+  uint64_t n = 0;
+  for (const Piece &x : b) n += x.len;
+  return n;
+}
 
 struct Run {
   const wjq::Program &prog;
-  const uint8_t *data;
-  uint64_t n;
   Format fmt;
   OnInvalid oi;
   warpjq_ctx *ctx = nullptr;
@@ -293,7 +294,7 @@ struct Run {
   bool csv_header_done = false;
 };
 
-static int drain(Run &r, uint32_t slot, const Chunk &c, std::vector<Chunk> &redo) {
+static int drain(Run &r, uint32_t slot, const Batch &b, std::vector<Batch> &redo) {
   warpjq_chunk_result res;
   char e[512] = {0};
   warpjq_status s = warpjq_wait(r.ctx, slot, &res, e, sizeof e);
@@ -307,25 +308,35 @@ static int drain(Run &r, uint32_t slot, const Chunk &c, std::vector<Chunk> &redo
   r.st.kernel_ms += kms;
   r.st.chunks++;
   if (res.chunk_overflow || res.group_overflow) {
+    r.st.gpu_redone_chunks++;
+    if (b.size() > 1) {  // halve the piece list
+      const size_t h = b.size() / 2;
+      redo.push_back(Batch(b.begin(), b.begin() + h));
+      redo.push_back(Batch(b.begin() + h, b.end()));
+      return 0;
+    }
     // Split at the newline nearest the middle and run both halves again.
-    const uint8_t *d = r.data + c.off;
-    uint64_t mid = c.len / 2;
-    const uint8_t *nl = (const uint8_t *)memchr(d + mid, '\n', c.len - mid);
-    uint64_t cut = nl ? (uint64_t)(nl - d) + 1 : c.len;
-    if (cut >= c.len) {
+    const uint8_t *d = b[0].p;
+    const uint64_t len = b[0].len;
+    uint64_t mid = len / 2;
+    const uint8_t *nl = (const uint8_t *)memchr(d + mid, '\n', len - mid);
+    uint64_t cut = nl ? (uint64_t)(nl - d) + 1 : len;
+    if (cut >= len) {
       nl = (const uint8_t *)memrchr(d, '\n', mid);
       cut = nl ? (uint64_t)(nl - d) + 1 : 0;
     }
-    if (cut == 0 || cut >= c.len) {
+    if (cut == 0 || cut >= len) {
       r.err = "a single line exceeds the device's per-chunk capacity";
       return 3;
     }
-    r.st.gpu_redone_chunks++;
-    redo.push_back({c.off, cut});
-    redo.push_back({c.off + cut, c.len - cut});
+    const uint8_t *dd = b[0].d;
+    redo.push_back(Batch{{d, cut, dd}});
+    redo.push_back(Batch{{d + cut, len - cut, dd ? dd + cut : nullptr}});
     return 0;
   }
-  const uint8_t *data = r.data + c.off;
+  // warpjq_submit: key offsets are relative to the chunk; a batched submit:
+  // they index the slot's key copy
+  const uint8_t *data = (b.size() == 1 && !b[0].d) ? b[0].p : warpjq_amd_group_keys(r.ctx, slot);
   uint64_t first_line = r.lines_seen + 1;
   r.lines_seen += res.n_lines;
   r.st.lines_in += res.n_lines - res.n_blank;
@@ -422,19 +433,83 @@ static bool next_chunk(const uint8_t *d, uint64_t n, uint64_t &pos, uint64_t cap
   return true;
 }
 
-static int submit(Run &r, uint32_t slot, const Chunk &c) {
+static int submit(Run &r, uint32_t slot, const Batch &b) {
   uint8_t *buf = warpjq_slot_buffer(r.ctx, slot);
-  memcpy(buf, r.data + c.off, c.len);
   char e[512] = {0};
-  if (warpjq_submit(r.ctx, slot, c.len, e, sizeof e) != WARPJQ_OK) {
+  warpjq_status st;
+  uint64_t n = 0;
+  if (b[0].d) {  // resident: no copy, one launch over the spans
+    std::vector<warpjq_amd_span> sp(b.size());
+    for (size_t i = 0; i < b.size(); i++) {
+      sp[i] = {b[i].d, b[i].len};
+      n += b[i].len;
+    }
+    st = warpjq_amd_submit_spans(r.ctx, slot, sp.data(), (uint32_t)sp.size(), e, sizeof e);
+  } else if (b.size() == 1) {  // the reference ABI's own submit
+    memcpy(buf, b[0].p, b[0].len);
+    n = b[0].len;
+    st = warpjq_submit(r.ctx, slot, n, e, sizeof e);
+  } else {  // packed back to back; one copy and one launch for all pieces
+    std::vector<warpjq_amd_piece> pcs(b.size());
+    for (size_t i = 0; i < b.size(); i++) {
+      memcpy(buf + n, b[i].p, b[i].len);
+      pcs[i] = {n, b[i].len};
+      n += b[i].len;
+    }
+    st = warpjq_amd_submit_pieces(r.ctx, slot, n, pcs.data(), (uint32_t)pcs.size(), e, sizeof e);
+  }
+  if (st != WARPJQ_OK) {
     r.err = std::string("submitting a chunk: ") + e;
     return 3;
   }
-  r.st.bytes_in += c.len;
+  r.st.bytes_in += n;
   return 0;
 }
 
-}  // namespace
+bool Batcher::next(Batch &b, std::string &err) {
+  b.clear();
+  uint64_t used = 0, lines = 0;
+  while (fi < n_files) {
+    const uint8_t *d = files[fi];
+    const uint64_t n = lens[fi];
+    if (!started) {
+      started = true;
+      pos = (n >= 3 && d[0] == 0xEF && d[1] == 0xBB && d[2] == 0xBF) ? 3 : 0;
+    }
+    if (pos >= n) {
+      fi++;
+      started = false;
+      continue;
+    }
+    if (b.size() >= max_pieces) break;
+    const uint64_t room = cap - used;
+    // a large rest starts a batch of its own rather than being cut small
+    if (!b.empty() && n - pos > room && room < cap / 4) break;
+    uint64_t p = pos;
+    Chunk c;
+    std::string e;
+    if (!next_chunk(d, n, p, room, max_lines, max_line, c, e)) {
+      if (!b.empty()) break;  // no newline within the room left: next batch
+      err = e.empty() ? "a line is longer than the device chunk" : e;
+      return false;
+    }
+    // line budget for the whole batch (the kernels report an overflow
+    // otherwise, and the batch is split)
+    uint64_t sample = std::min<uint64_t>(1 << 16, c.len), seen = 0;
+    for (uint64_t i = 0; i < sample; i++) seen += d[c.off + i] == '\n';
+    const uint64_t est = sample ? seen * c.len / sample + 1 : 1;
+    if (!b.empty() && (lines + est) * 2 >= max_lines) break;
+    lines += est;
+    b.push_back({d + c.off, c.len, dfiles ? dfiles[fi] + c.off : nullptr});
+    used += c.len;
+    pos = p;
+    if (pos < n || used >= cap) break;  // an input that was cut ends its batch
+  }
+  return !b.empty();
+}
+
+
+}  // namespace wjqd
 
 struct warpjq_amd_program {
   wjq::Program prog;
@@ -463,12 +538,35 @@ const warpjq_program *warpjq_amd_program_view(const warpjq_amd_program *p) { ret
 
 void warpjq_amd_program_free(warpjq_amd_program *p) { delete p; }
 
-int warpjq_amd_run_buffer(const char *query, const uint8_t *data, uint64_t n, int format, int on_invalid,
-                          uint64_t chunk_bytes, uint64_t max_line_bytes, uint8_t **out, uint64_t *out_len,
-                          warpjq_amd_stats *stats, char *err, size_t err_len) {
-  *out = nullptr;
-  *out_len = 0;
-  memset(stats, 0, sizeof *stats);
+}  // extern "C"
+
+namespace wjqd {
+
+int engine_open(Engine &g, const char *query, int format, uint64_t chunk_bytes, char *err, size_t err_len) {
+  wjq::QueryError qe;
+  if (!wjq::parse(query, g.prog, qe)) {
+    snprintf(err, err_len, "%s", qe.render(query).c_str());
+    return 1;
+  }
+  if (format == F_CSV && g.prog.output.kind == wjq::Output::PASSTHROUGH) {  // exec/mod.rs:256-266
+    snprintf(err, err_len, "--csv needs a query with named columns, but this one emits whole lines");
+    return 3;
+  }
+  g.csv = format == F_CSV;
+  wjq::lower(g.prog, g.csv, g.low);
+  g.view = g.low.view();
+  g.cap = std::max<uint64_t>(1, std::min<uint64_t>(chunk_bytes, 64ull << 20));
+  char e[512] = {0};
+  if (warpjq_ctx_create(&g.view, g.cap, 2, &g.ctx, e, sizeof e) != WARPJQ_OK) {
+    snprintf(err, err_len, "creating the device context: %s", e);
+    g.ctx = nullptr;
+    return 3;
+  }
+  g.max_lines = warpjq_max_lines(g.ctx);
+  return 0;
+}
+
+int engine_reprogram(Engine &g, const char *query, int format, char *err, size_t err_len) {
   wjq::Program prog;
   wjq::QueryError qe;
   if (!wjq::parse(query, prog, qe)) {
@@ -479,21 +577,43 @@ int warpjq_amd_run_buffer(const char *query, const uint8_t *data, uint64_t n, in
     snprintf(err, err_len, "--csv needs a query with named columns, but this one emits whole lines");
     return 3;
   }
+  if (prog.is_aggregate() != g.prog.is_aggregate() || bool(prog.group_by) != bool(g.prog.group_by)) return 2;
+  const bool csv = format == F_CSV;
   wjq::Lowered low;
-  wjq::lower(prog, format == F_CSV, low);
+  wjq::lower(prog, csv, low);
   warpjq_program view = low.view();
-  Run r{prog, data, n, (Format)format, (OnInvalid)on_invalid};
-  // BOM (chunk.rs:210-212)
-  uint64_t pos = 0;
-  if (n >= 3 && data[0] == 0xEF && data[1] == 0xBB && data[2] == 0xBF) pos = 3;
-  r.cap = std::max<uint64_t>(1, std::min<uint64_t>(chunk_bytes, 64ull << 20));
-  if (r.cap < (1u << 20)) r.cap = std::min<uint64_t>(chunk_bytes, r.cap);
   char e[512] = {0};
-  if (warpjq_ctx_create(&view, r.cap, 2, &r.ctx, e, sizeof e) != WARPJQ_OK) {
-    snprintf(err, err_len, "creating the device context: %s", e);
-    return 3;
+  if (warpjq_amd_ctx_reprogram(g.ctx, &view, e, sizeof e) != WARPJQ_OK) return 2;
+  g.prog = std::move(prog);
+  g.low = std::move(low);
+  g.view = g.low.view();
+  g.csv = csv;
+  return 0;
+}
+
+void engine_close(Engine &g) {
+  if (g.ctx) warpjq_ctx_destroy(g.ctx);
+  g.ctx = nullptr;
+}
+
+int engine_run(Engine &g, Batcher &src, int format, int on_invalid, uint8_t **out, uint64_t *out_len,
+               warpjq_amd_stats *stats, char *err, size_t err_len) {
+  *out = nullptr;
+  *out_len = 0;
+  memset(stats, 0, sizeof *stats);
+  const wjq::Program &prog = g.prog;
+  Run r{prog, (Format)format, (OnInvalid)on_invalid};
+  r.ctx = g.ctx;
+  r.cap = g.cap;
+  r.max_lines = g.max_lines;
+  src.cap = r.cap;
+  src.max_lines = r.max_lines;
+  if (src.dfiles && prog.is_aggregate()) {
+    // resident input, aggregate: one launch takes up to 3.5 GiB of spans
+    // (positions stay under 4 GiB with every span rounded up to a claim)
+    src.cap = 7ull << 29;
+    src.max_lines = ~0ull >> 2;
   }
-  r.max_lines = warpjq_max_lines(r.ctx);
 
   if (r.fmt == F_CSV) {  // write_header
     if (auto cols = prog.csv_headers()) {
@@ -505,21 +625,31 @@ int warpjq_amd_run_buffer(const char *query, const uint8_t *data, uint64_t n, in
     }
   }
 
-  std::vector<Chunk> pending;  // chunks to resubmit after an overflow, in order
+  bool inflight[2] = {false, false};
+  auto put = [&](uint32_t sl, const Batch &b) {
+    int rc2 = submit(r, sl, b);
+    if (!rc2) inflight[sl] = true;
+    return rc2;
+  };
+  auto take = [&](uint32_t sl, const Batch &b, std::vector<Batch> &redo) {
+    inflight[sl] = false;
+    return drain(r, sl, b, redo);
+  };
+  std::vector<Batch> pending;  // batches to resubmit after an overflow, in order
   bool have_prev = false;
-  Chunk prev{};
+  Batch prev;
   uint32_t slot = 0;
   int rc = 0;
   std::string cerr;
   for (;;) {
-    Chunk c;
+    Batch c;
     bool got;
     if (!pending.empty()) {
       c = pending.front();
       pending.erase(pending.begin());
       got = true;
     } else {
-      got = next_chunk(data, n, pos, r.cap, r.max_lines, max_line_bytes, c, cerr);
+      got = src.next(c, cerr);
       if (!got && !cerr.empty()) {
         rc = 3;
         r.err = cerr;
@@ -527,20 +657,18 @@ int warpjq_amd_run_buffer(const char *query, const uint8_t *data, uint64_t n, in
       }
     }
     if (!got) break;
-    if (have_prev && !pending.empty()) {
-      // only reached after an overflow redo was queued; keep order strict
-    }
-    if ((rc = submit(r, slot, c))) break;
+    if ((rc = put(slot, c))) break;
     if (have_prev) {
-      std::vector<Chunk> redo;
-      if ((rc = drain(r, slot ^ 1, prev, redo))) break;
+      std::vector<Batch> redo;
+      if ((rc = take(slot ^ 1, prev, redo))) break;
       if (!redo.empty()) {
-        // the previous chunk must be redone before this one: drain this one
-        // too, discard it, and queue redo + this chunk again
+        // the previous batch must be redone before this one: drain this one
+        // too, discard it, and queue redo + this batch again
         warpjq_chunk_result tmp;
         char e2[512];
         warpjq_wait(r.ctx, slot, &tmp, e2, sizeof e2);
-        r.st.bytes_in -= prev.len + c.len;
+        inflight[slot] = false;
+        r.st.bytes_in -= batch_bytes(prev) + batch_bytes(c);
         redo.push_back(c);
         pending.insert(pending.begin(), redo.begin(), redo.end());
         have_prev = false;
@@ -552,24 +680,29 @@ int warpjq_amd_run_buffer(const char *query, const uint8_t *data, uint64_t n, in
     slot ^= 1;
   }
   if (!rc && have_prev) {
-    std::vector<Chunk> redo;
-    rc = drain(r, slot ^ 1, prev, redo);
-    while (!rc && !redo.empty()) {
-      r.st.bytes_in -= prev.len;
-      std::vector<Chunk> todo = redo;
-      redo.clear();
+    std::vector<Batch> redo;
+    rc = take(slot ^ 1, prev, redo);
+    if (!rc && !redo.empty()) {
+      r.st.bytes_in -= batch_bytes(prev);
+      std::vector<Batch> todo = redo;
       for (size_t i = 0; i < todo.size() && !rc; i++) {
-        std::vector<Chunk> more;
-        if ((rc = submit(r, 0, todo[i]))) break;
-        rc = drain(r, 0, todo[i], more);
+        std::vector<Batch> more;
+        if ((rc = put(0, todo[i]))) break;
+        rc = take(0, todo[i], more);
         if (!rc && !more.empty()) {
-          r.st.bytes_in -= todo[i].len;
+          r.st.bytes_in -= batch_bytes(todo[i]);
           todo.insert(todo.begin() + i + 1, more.begin(), more.end());
         }
       }
     }
   }
-  warpjq_ctx_destroy(r.ctx);
+  // leave no slot in flight: the context is reused by the next run
+  for (uint32_t sl = 0; sl < 2; sl++)
+    if (inflight[sl]) {
+      warpjq_chunk_result tmp;
+      char e2[512];
+      warpjq_wait(r.ctx, sl, &tmp, e2, sizeof e2);
+    }
   stats->bytes_in = r.st.bytes_in;
   stats->lines_in = r.st.lines_in;
   stats->malformed = r.st.malformed;
@@ -622,9 +755,50 @@ int warpjq_amd_run_buffer(const char *query, const uint8_t *data, uint64_t n, in
   if (r.fmt == F_COUNT) r.out = std::to_string(r.rows) + "\n";
   stats->lines_out = r.rows;
   *out = (uint8_t *)malloc(r.out.size() + 1);
+  if (!*out) {
+    snprintf(err, err_len, "out of host memory for the output");
+    return 3;
+  }
   memcpy(*out, r.out.data(), r.out.size());
   *out_len = r.out.size();
   return 0;
+}
+
+}  // namespace wjqd
+
+extern "C" {
+
+static int run_once(const char *query, wjqd::Batcher &src, int format, int on_invalid, uint64_t chunk_bytes,
+                    uint8_t **out, uint64_t *out_len, warpjq_amd_stats *stats, char *err, size_t err_len) {
+  *out = nullptr;
+  *out_len = 0;
+  memset(stats, 0, sizeof *stats);
+  wjqd::Engine g;
+  int rc = wjqd::engine_open(g, query, format, chunk_bytes, err, err_len);
+  if (rc) return rc;
+  rc = wjqd::engine_run(g, src, format, on_invalid, out, out_len, stats, err, err_len);
+  wjqd::engine_close(g);
+  return rc;
+}
+
+int warpjq_amd_run_buffer(const char *query, const uint8_t *data, uint64_t n, int format, int on_invalid,
+                          uint64_t chunk_bytes, uint64_t max_line_bytes, uint8_t **out, uint64_t *out_len,
+                          warpjq_amd_stats *stats, char *err, size_t err_len) {
+  wjqd::Batcher src{&data, &n, 1, 0, 0, max_line_bytes, 1};
+  return run_once(query, src, format, on_invalid, chunk_bytes, out, out_len, stats, err, err_len);
+}
+
+int warpjq_amd_run_files(const char *query, const uint8_t *const *files, const uint64_t *lens, uint32_t n_files,
+                         int format, int on_invalid, uint64_t chunk_bytes, uint64_t max_line_bytes, uint8_t **out,
+                         uint64_t *out_len, warpjq_amd_stats *stats, char *err, size_t err_len) {
+  if (n_files && (!files || !lens)) {
+    *out = nullptr;
+    *out_len = 0;
+    snprintf(err, err_len, "run_files: null file list");
+    return 3;
+  }
+  wjqd::Batcher src{files, lens, n_files, 0, 0, max_line_bytes, 4096};
+  return run_once(query, src, format, on_invalid, chunk_bytes, out, out_len, stats, err, err_len);
 }
 
 }  // extern "C"

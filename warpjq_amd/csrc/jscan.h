@@ -23,6 +23,8 @@
 
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../../include/warpjq_kernels.h"
 
 #if defined(__HIPCC__)
@@ -153,8 +155,16 @@ WJQ_HD uint32_t skip_ws(R &rd, uint32_t i) {
 // '"', '\\' or byte < 0x20 at or after j, or a byte before it that
 // str_special flags falsely after a hit earlier in the same dword (callers
 // re-check the byte). Every line ends in '\n', so it exists.
+// A reader whose every lane reads the same positions in step can search
+// 64 blocks at once (next_special_wide).
+template <class R, class = void>
+struct wide_reader : std::false_type {};
+template <class R>
+struct wide_reader<R, std::void_t<decltype(&R::next_special_wide)>> : std::true_type {};
+
 template <class R>
 WJQ_HD uint32_t next_special(R &rd, uint32_t j) {
+  if constexpr (wide_reader<R>::value) return rd.next_special_wide(j);
   for (;;) {
     Blk16 v = rd.block(j >> 4);
     uint32_t m = movemask16(str_special(v.x), str_special(v.y), str_special(v.z), str_special(v.w)) &

@@ -31,10 +31,14 @@
 //   D2H of the fixed-size results (counters, aggregate)
 // warpjq_wait synchronises the stream and copies the variable-size results.
 //
-// Positions inside the kernels are "q positions": offsets from the chunk's
-// 16-byte-aligned-down base, so the chunk's first byte is at q = mis (its
-// misalignment, 0..15). All 16-byte loads are then aligned, and arena offsets
-// agree with q positions modulo 16.
+// Positions inside the kernels are virtual "q positions". A submit is a list
+// of spans (one chunk, or many files / newline-complete pieces of files for
+// the batched form); span s is placed at q = vbase_s, a multiple of the claim
+// size, with its 16-byte-aligned-down base there, so its first byte is at
+// start_s = vbase_s + (its misalignment) and every claim lies in one span.
+// All 16-byte loads are then aligned, arena offsets agree with q positions
+// modulo 16, and bytes past a span's end read as '\n' (each span's last line
+// ends there, as at the end of a file).
 //
 // Reference ABI semantics followed: kernels.cu:2233-2685 (probe, abi_check,
 // ctx_create sizing, submit, wait), kernels.h:155-230 (result meaning).
@@ -80,6 +84,8 @@ using namespace wjqd;
 #define ROWSCAN_BLOCK 1024u          // lines per block in the row-offset scan
 #define EMPTY_KEY (~0ull)
 #define TAB_LDS_MAX 16384u           // program tables staged in LDS per workgroup
+#define MAX_SPANS 65536u             // spans per submit
+#define KEYBLOB_BYTES (16u << 20)    // group-key bytes returned by a multi-span submit
 
 // ---------------------------------------------------------------------------
 // device-side structures
@@ -91,6 +97,8 @@ struct DevCtr {
   unsigned int first_invalid, overflow, saw_non_numeric, group_overflow;
   unsigned int fb_count, n_groups, ticket, tiles_done;
   unsigned int rticket[8];  // aggregate queries: one ticket per region of the chunk
+  unsigned int blob_used;   // bytes of group keys copied out (multi-span submits)
+  unsigned int long_count, long_next;  // deferred long lines: added, taken
 };
 
 // Per-wave ungrouped aggregate partial.
@@ -109,10 +117,18 @@ struct GroupTab {
   unsigned int cap;  // power of two
 };
 
+// One span of a submit (see above). Spans are sorted by vbase.
+struct Span {
+  const uint8_t *base;  // 16-byte-aligned-down address of its first byte
+  uint32_t vbase;       // q position of `base`
+  uint32_t start;       // q position of its first byte
+  uint32_t vend;        // q position one past its last byte
+  uint32_t _pad;
+};
+
 struct KArgs {
-  const uint8_t *d;  // 16-byte-aligned base; the chunk starts at d + mis
-  uint32_t mis;
-  uint32_t nq;       // mis + chunk bytes
+  const Span *spans;
+  uint32_t n_spans;
   uint32_t ntiles;
   uint32_t claim;    // bytes per claim (multiple of STEP_BYTES)
   uint32_t arena;    // LDS arena bytes per wave (multiple of 256)
@@ -131,6 +147,10 @@ struct KArgs {
   // declined lines (ST_FALLBACK)
   uint32_t *fb_idx, *fb_off, *fb_len, *fb_tile;
   uint32_t fb_cap;
+  // lines too long to stage with their claim, walked after the claims:
+  // {q start, rec, tile + 1} each, the last word written last (0 = not yet)
+  uint32_t *long_q;
+  uint32_t long_cap;
 };
 
 // ---------------------------------------------------------------------------
@@ -171,6 +191,23 @@ struct DevReader {
   __device__ __forceinline__ uint32_t byte(uint32_t i) { return (dword(i >> 2) >> ((i & 3) * 8)) & 0xFFu; }
 };
 
+// Index of the span holding q position q (spans sorted by vbase).
+__device__ __forceinline__ uint32_t span_of(const Span *sp, uint32_t n, uint32_t q) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (sp[mid].vbase <= q) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// A reader over the span holding q position q.
+__device__ __forceinline__ DevReader span_reader(const Span *sp, uint32_t n, uint32_t q) {
+  const Span &s = sp[span_of(sp, n, q)];
+  return DevReader(s.base - s.vbase, s.vend);
+}
+
 // Reads a line staged in the wave's LDS arena: position i lives at arena
 // byte i + delta (delta is a multiple of 16, so blocks are aligned). Keeps no
 // state, so nothing it holds has to follow the walker's control flow: every
@@ -191,6 +228,74 @@ struct LdsReader {
     return *(const LDS_AS uint32_t *)(b + ((k << 2) + delta));
   }
   __device__ __forceinline__ uint32_t byte(uint32_t i) const { return b[i + delta]; }
+};
+
+// Reads one line through a window of the wave's LDS arena that the whole
+// wave refills with direct-to-LDS copies (64 x 16 bytes per instruction, one
+// wait per window), instead of one dependent HBM load per 16 bytes. Every
+// lane must read the same positions in step: the long-line phase walks one
+// line with all 64 lanes. Positions at or past n read as '\n'.
+struct WinReader {
+  const uint8_t *d;
+  uint32_t n;
+  LDS_AS uint8_t *w;
+  uint32_t wblk;  // window size in 16-byte blocks (a multiple of 64)
+  uint32_t wb;    // first block held
+
+  __device__ WinReader(const uint8_t *d_, uint32_t n_, LDS_AS uint8_t *w_, uint32_t wbytes)
+      : d(d_), n(n_), w(w_), wblk(wbytes >> 4), wb(0x80000000u) {}
+
+  // The window from block b on (every lane takes part).
+  __device__ __forceinline__ void fill(uint32_t b) {
+    wb = b;
+    const uint32_t p0 = b << 4;
+    const uint32_t lane = __lane_id();
+    for (uint32_t j = 0; j < (wblk << 4); j += 1024) {
+      const uint32_t x = j + lane * 16u;
+      if (p0 + x < n) __builtin_amdgcn_global_load_lds((const void *)(d + p0 + x), (LDS_AS void *)(w + j), 16, 0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+  }
+  __device__ __forceinline__ Blk16 block(uint32_t b) {
+    if (b - wb >= wblk) fill(b);
+    wjq_v4u x = *(const LDS_AS wjq_v4u *)(w + ((b - wb) << 4));
+    Blk16 v{x.x, x.y, x.z, x.w};
+    const uint32_t p = b << 4;
+    if (p + 15 >= n) {
+      uint32_t q4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (uint32_t q = 0; q < 16; q++)
+        if (p + q >= n) q4[q >> 2] = (q4[q >> 2] & ~(0xFFu << (8 * (q & 3)))) | (WJQ_END << (8 * (q & 3)));
+      v = Blk16{q4[0], q4[1], q4[2], q4[3]};
+    }
+    return v;
+  }
+  __device__ __forceinline__ uint32_t dword(uint32_t k) {
+    Blk16 x = block(k >> 2);
+    uint32_t s = k & 3;
+    return s == 0 ? x.x : s == 1 ? x.y : s == 2 ? x.z : x.w;
+  }
+  __device__ __forceinline__ uint32_t byte(uint32_t i) { return (dword(i >> 2) >> ((i & 3) * 8)) & 0xFFu; }
+  // First position >= j holding a byte a string scan must look at: lane k
+  // tests block (j >> 4) + k, 1 KiB per step, and the lowest flagged lane
+  // gives the answer to all.
+  __device__ __forceinline__ uint32_t next_special_wide(uint32_t j) {
+    const uint32_t lane = __lane_id();
+    uint32_t b0 = j >> 4;
+    for (;;) {
+      if (b0 - wb >= wblk || b0 + 64u - wb > wblk) fill(b0);
+      Blk16 v = block(b0 + lane);
+      uint32_t m = movemask16(str_special(v.x), str_special(v.y), str_special(v.z), str_special(v.w));
+      if (lane == 0) m &= 0xFFFFu << (j & 15u);
+      const uint64_t bal = __ballot(m != 0);
+      if (bal) {
+        const uint32_t L = (uint32_t)__builtin_ctzll(bal);
+        const uint32_t mL = __shfl(m, (int)L, 64);
+        return ((b0 + L) << 4) + (uint32_t)__builtin_ctz(mL);
+      }
+      b0 += 64;
+    }
+  }
 };
 
 // The program tables staged in LDS: the same fields as DProg, read with
@@ -427,7 +532,8 @@ __device__ __forceinline__ void group_apply(const GroupTab &g, uint32_t pos, con
 // on overflow. Stored keys are read from HBM through `gr`. The key is
 // published whole by one CAS, so a reader never sees a half-written key.
 template <class RK>
-__device__ __forceinline__ uint32_t group_find(RK &rk, DevReader &gr, const GroupTab &g, const GKey &k, uint32_t h) {
+__device__ __forceinline__ uint32_t group_find(RK &rk, const Span *spans, uint32_t n_spans, const GroupTab &g,
+                                               const GKey &k, uint32_t h) {
   unsigned long long pk = pack_key(k);
   uint32_t pos = h & (g.cap - 1);
   for (uint32_t probe = 0; probe < GROUP_MAX_PROBE; probe++) {
@@ -442,7 +548,10 @@ __device__ __forceinline__ uint32_t group_find(RK &rk, DevReader &gr, const Grou
     }
     unsigned int hv = __hip_atomic_load(&g.hash[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (hv == 0 || hv == h) {
-      if (cur == pk || group_eq(rk, k, gr, unpack_key(cur))) return pos;
+      if (cur == pk) return pos;
+      const GKey o = unpack_key(cur);
+      DevReader gr = span_reader(spans, n_spans, o.off);
+      if (group_eq(rk, k, gr, o)) return pos;
     }
     pos = (pos + 1) & (g.cap - 1);
   }
@@ -467,8 +576,9 @@ __device__ __forceinline__ void lds_acc_f64(double *s, double *c, double v) {
 
 // Inserts one line's value; falls through to HBM when the LDS table is full.
 template <class R>
-__device__ __forceinline__ void group_line(R &rd, DevReader &gr, const GroupTab &g, LGroup &L, const GKey &k, uint32_t h,
-                           bool has_num, bool is_int, int64_t iv, double v, unsigned int &ovf) {
+__device__ __forceinline__ void group_line(R &rd, const Span *spans, uint32_t n_spans, const GroupTab &g, LGroup &L,
+                                           const GKey &k, uint32_t h, bool has_num, bool is_int, int64_t iv, double v,
+                                           unsigned int &ovf) {
   unsigned long long pk = pack_key(k);
   uint32_t pos = h & (LGROUP - 1);
   int found = -1;
@@ -484,9 +594,17 @@ __device__ __forceinline__ void group_line(R &rd, DevReader &gr, const GroupTab 
       cur = old;
     }
     unsigned int hv = L.hash[pos];
-    if ((hv == 0 || hv == h) && (cur == pk || group_eq(rd, k, gr, unpack_key(cur)))) {
-      found = (int)pos;
-      break;
+    if (hv == 0 || hv == h) {
+      bool same = cur == pk;
+      if (!same) {
+        const GKey o = unpack_key(cur);
+        DevReader gr = span_reader(spans, n_spans, o.off);
+        same = group_eq(rd, k, gr, o);
+      }
+      if (same) {
+        found = (int)pos;
+        break;
+      }
     }
     pos = (pos + 1) & (LGROUP - 1);
   }
@@ -518,7 +636,7 @@ __device__ __forceinline__ void group_line(R &rd, DevReader &gr, const GroupTab 
     }
     return;
   }
-  uint32_t gp = group_find(rd, gr, g, k, h);
+  uint32_t gp = group_find(rd, spans, n_spans, g, k, h);
   if (gp == ~0u) {
     ovf = 1;
     return;
@@ -548,12 +666,48 @@ struct LaneAcc {
   unsigned int ovf, govf;
 };
 
+__device__ __forceinline__ void acc_zero(LaneAcc &L) {
+  L.blank = L.inv = L.te = L.fb = L.sel = 0;
+  L.cnt = L.num = L.ilo = L.ihi = 0;
+  L.fs = L.fc = 0.0;
+  L.mn = INFINITY;
+  L.mx = -INFINITY;
+  L.saw_nn = false;
+  L.ovf = L.govf = 0;
+}
+
+// L += d (d's float part added with the same compensation as one value).
+__device__ __forceinline__ void acc_merge(LaneAcc &L, const LaneAcc &d) {
+  L.blank += d.blank;
+  L.inv += d.inv;
+  L.te += d.te;
+  L.fb += d.fb;
+  L.sel += d.sel;
+  L.cnt += d.cnt;
+  L.num += d.num;
+  const uint64_t old = L.ilo;
+  L.ilo += d.ilo;
+  L.ihi += d.ihi + (L.ilo < old ? 1 : 0);
+  const double t = L.fs + d.fs;
+  if (fabs(L.fs) >= fabs(d.fs)) L.fc += (L.fs - t) + d.fs;
+  else L.fc += (d.fs - t) + L.fs;
+  L.fs = t;
+  L.fc += d.fc;
+  if (d.mn < L.mn) L.mn = d.mn;
+  if (d.mx > L.mx) L.mx = d.mx;
+  L.saw_nn = L.saw_nn || d.saw_nn;
+  L.ovf |= d.ovf;
+  L.govf |= d.govf;
+}
+
 // One line: evaluate, then fold its outcome into the lane's tallies, the
 // group table or the per-line row arrays. `rec` is the line's index in the
 // chunk (row outputs) or its rank in claim `tile` (aggregates).
+// `act` false: evaluate and tally into L only (the long-line phase runs one
+// line on every lane; lane 0 alone touches shared state).
 template <int NS, bool GRP, bool LB, class R>
 __device__ __forceinline__ void process_line(R &rd, const KArgs &a, const LProg &P, uint32_t s, uint32_t rec,
-                                             uint32_t tile, LaneAcc &L, LGroup *lgp) {
+                                             uint32_t tile, LaneAcc &L, LGroup *lgp, bool act = true) {
   SlotV sl[NS];
   LineResult r = eval_line<NS>(rd, s, P, sl);
   if (r.status == ST_OK && r.selected && P.output_kind == WARPJQ_OUT_AGG) {
@@ -586,8 +740,7 @@ __device__ __forceinline__ void process_line(R &rd, const KArgs &a, const LProg 
     if (undecided) {
       r.status = ST_FALLBACK;
     } else if (GRP) {
-      DevReader gr(a.d, a.nq);
-      group_line(rd, gr, a.gt, *lgp, gk, gh, has_num, is_int, iv, v, L.govf);
+      if (act) group_line(rd, a.spans, a.n_spans, a.gt, *lgp, gk, gh, has_num, is_int, iv, v, L.govf);
     } else {
       L.cnt++;
       if (has_num) {
@@ -616,8 +769,10 @@ __device__ __forceinline__ void process_line(R &rd, const KArgs &a, const LProg 
       L.sel++;
     }
     if (rec < a.max_lines) {
-      a.row_len[rec] = rl;
-      if (rl) a.sel_start[rec] = s;
+      if (act) {
+        a.row_len[rec] = rl;
+        if (rl) a.sel_start[rec] = s;
+      }
     } else {
       L.ovf = 1;
     }
@@ -626,8 +781,10 @@ __device__ __forceinline__ void process_line(R &rd, const KArgs &a, const LProg 
     case ST_BLANK: L.blank++; break;
     case ST_INVALID:
       L.inv++;
-      if (LB) atomicMin(&a.ctr->first_invalid, rec);
-      else atomicMin(&a.ctr->first_bad, ((unsigned long long)tile << 32) | rec);
+      if (act) {
+        if (LB) atomicMin(&a.ctr->first_invalid, rec);
+        else atomicMin(&a.ctr->first_bad, ((unsigned long long)tile << 32) | rec);
+      }
       break;
     case ST_TYPE_ERROR: L.te++; break;
     case ST_FALLBACK: {
@@ -636,16 +793,18 @@ __device__ __forceinline__ void process_line(R &rd, const KArgs &a, const LProg 
       while (rd.byte(e) != WJQ_END) e++;
       uint32_t len = e - s;
       if (len > 0 && rd.byte(e - 1) == '\r') len--;
-      uint32_t k = atomicAdd(&a.ctr->fb_count, 1u);
-      if (k < a.fb_cap) {
-        a.fb_idx[k] = rec;
-        a.fb_tile[k] = tile;
-        a.fb_off[k] = s - a.mis;
-        a.fb_len[k] = len;
-      } else {
-        L.ovf = 1;
+      if (act) {
+        uint32_t k = atomicAdd(&a.ctr->fb_count, 1u);
+        if (k < a.fb_cap) {
+          a.fb_idx[k] = rec;
+          a.fb_tile[k] = tile;
+          a.fb_off[k] = s;  // q position; the host maps it back through the spans
+          a.fb_len[k] = len;
+        } else {
+          L.ovf = 1;
+        }
+        if (LB && rec < a.max_lines) a.row_len[rec] = 0;
       }
-      if (LB && rec < a.max_lines) a.row_len[rec] = 0;
       break;
     }
     default: break;
@@ -692,6 +851,7 @@ struct LQueue {
 // The claim whose line starts are still being queued.
 struct Claim {
   uint32_t t, t0, end;  // claim index, first q position, end of its line starts
+  uint32_t start;       // q position of its span's first byte
   uint32_t slot;        // its claim-ring slot
   uint32_t at;          // arena offset of q position t0
   uint32_t resume;      // first line start not yet queued (q position)
@@ -725,9 +885,9 @@ __device__ __forceinline__ uint32_t enqueue_lines(const KArgs &a, const LDS_AS u
       const uint32_t m = nl_mask16(lds_ld16(base + p));
       uint32_t x = ((m << 1) | prev) & 0xFFFFu;
       prev = m >> 15;
-      // the chunk's first line starts at q = mis; nothing before it does
-      if (p + 16u <= a.mis) x = 0;
-      else if (p <= a.mis) x = (x & (0xFFFFu << (a.mis - p))) | (1u << (a.mis - p));
+      // a span's first line starts at its first byte; nothing before it does
+      if (p + 16u <= c.start) x = 0;
+      else if (p <= c.start) x = (x & (0xFFFFu << (c.start - p))) | (1u << (c.start - p));
       // keep starts in [c.resume, c.end)
       if (p + 16u <= c.resume) x = 0;
       else if (p < c.resume) x &= 0xFFFFu << (c.resume - p);
@@ -805,13 +965,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   }
 
   LaneAcc L;
-  L.blank = L.inv = L.te = L.fb = L.sel = 0;
-  L.cnt = L.num = L.ilo = L.ihi = 0;
-  L.fs = L.fc = 0.0;
-  L.mn = INFINITY;
-  L.mx = -INFINITY;
-  L.saw_nn = false;
-  L.ovf = L.govf = 0;
+  acc_zero(L);
 
   uint64_t lines_total = 0;     // wave-uniform
   uint32_t qhead = 0, qlen = 0;  // wave-uniform
@@ -854,13 +1008,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   // byte (read through the scalar cache; waited for on use).
   auto issue_fetch = [&](uint32_t t, uint32_t at) -> uint32_t {
     const uint32_t t0 = t * a.claim;
+    const Span sp = a.spans[span_of(a.spans, a.n_spans, t0)];
+    const uint8_t *src = sp.base + (t0 - sp.vbase);
     for (uint32_t j = 0; j < a.claim + TAIL_PIECE; j += STEP_BYTES) {
       const uint32_t x = j + lane * 16u;
-      if (x < a.claim + TAIL_PIECE && t0 + x < a.nq)
-        __builtin_amdgcn_global_load_lds((const void *)(a.d + t0 + x), (LDS_AS void *)(ar + at + j), 16, 0, 0);
+      if (x < a.claim + TAIL_PIECE && t0 + x < sp.vend)
+        __builtin_amdgcn_global_load_lds((const void *)(src + x), (LDS_AS void *)(ar + at + j), 16, 0, 0);
     }
-    uint32_t carry0 = 1;
-    if (t0 > 0) carry0 = (*(const uint32_t *)(a.d + t0 - 4) >> 24) == '\n' ? 1u : 0u;
+    uint32_t carry0 = 1;  // a span's first claim: its first line start is placed by c.start
+    if (t0 > sp.vbase) carry0 = (*(const uint32_t *)(src - 4) >> 24) == '\n' ? 1u : 0u;
     return carry0;
   };
   // Keeps the bytes of queued lines at the front of the arena.
@@ -891,18 +1047,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   // its line starts.
   auto process_claim = [&](uint32_t t, uint32_t at, uint32_t carry0) {
     const uint32_t t0 = t * a.claim;
+    const Span sp = a.spans[span_of(a.spans, a.n_spans, t0)];
+    const uint32_t vend = sp.vend;
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     uint32_t span = a.claim + TAIL_PIECE;  // bytes staged for this claim
-    if (t0 + span > a.nq) {
+    if (t0 + span > vend) {
       for (uint32_t x = lane * 16u; x < span; x += STEP_BYTES) {
         const uint32_t p = t0 + x;
-        if (p + 16u > a.nq) lds_st16(ar + at + x, end_patch(lds_ld16(ar + at + x), p, a.nq));
+        if (p + 16u > vend) lds_st16(ar + at + x, end_patch(lds_ld16(ar + at + x), p, vend));
       }
       __syncthreads();
     }
     // the claim's last line must end inside the staged bytes
-    bool found = t0 + a.claim >= a.nq || ar[at + a.claim - 1] == '\n';
+    bool found = t0 + a.claim >= vend || ar[at + a.claim - 1] == '\n';
     uint32_t piece = a.claim;
     while (!found) {
       const uint32_t x = piece + lane * 16u;
@@ -912,22 +1070,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       piece += TAIL_PIECE;
       if (found || piece >= a.claim + TAIL_MAX) break;
       // one more piece
-      if (lane < TAIL_PIECE / 16u && t0 + piece + lane * 16u < a.nq)
-        __builtin_amdgcn_global_load_lds((const void *)(a.d + t0 + piece + lane * 16u),
+      if (lane < TAIL_PIECE / 16u && t0 + piece + lane * 16u < vend)
+        __builtin_amdgcn_global_load_lds((const void *)(sp.base + (t0 - sp.vbase) + piece + lane * 16u),
                                          (LDS_AS void *)(ar + at + piece), 16, 0, 0);
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       span = piece + TAIL_PIECE;
-      if (t0 + span > a.nq) {
+      if (t0 + span > vend) {
         const uint32_t xx = piece + lane * 16u;
-        if (lane < TAIL_PIECE / 16u && t0 + xx + 16u > a.nq)
-          lds_st16(ar + at + xx, end_patch(lds_ld16(ar + at + xx), t0 + xx, a.nq));
+        if (lane < TAIL_PIECE / 16u && t0 + xx + 16u > vend)
+          lds_st16(ar + at + xx, end_patch(lds_ld16(ar + at + xx), t0 + xx, vend));
         __syncthreads();
       }
     }
     c.t = t;
     c.t0 = t0;
-    c.end = min(t0 + a.claim, a.nq);
+    c.end = min(t0 + a.claim, vend);
+    c.start = sp.start;
     c.at = at;
     c.resume = t0;
     c.rank = 0;
@@ -1032,8 +1191,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
         LdsReader rd(ar, off - s);
         process_line<NS, GRP, LB>(rd, a, P, s, rec, tile, L, lgp);
       } else {
-        DevReader rd(a.d, a.nq);
-        process_line<NS, GRP, LB>(rd, a, P, s, rec, tile, L, lgp);
+        // too long to stage: walked by the whole wave after the claims
+        const uint32_t k = atomicAdd(&a.ctr->long_count, 1u);
+        if (k < a.long_cap) {
+          a.long_q[3 * k] = s;
+          a.long_q[3 * k + 1] = rec;
+          __threadfence();
+          __hip_atomic_store(&a.long_q[3 * k + 2], tile + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          DevReader rd = span_reader(a.spans, a.n_spans, s);
+          process_line<NS, GRP, LB>(rd, a, P, s, rec, tile, L, lgp);
+        }
       }
     }
     qhead = (qhead + take) & QMASK;
@@ -1044,6 +1212,48 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       hseq += (hs - hseq) & (CRING - 1u);
     }
     __syncthreads();
+  }
+
+  // ---- long lines: one at a time, every lane walking it in step through
+  // the arena (free now) as a window. Each line is taken by some wave: the
+  // wave that added it looks here after adding it.
+  {
+    const uint32_t wbytes = a.arena & ~1023u;
+    for (;;) {
+      uint32_t k = ~0u;
+      if (lane == 0) {
+        for (;;) {
+          const uint32_t nx = __hip_atomic_load(&a.ctr->long_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const uint32_t ct = __hip_atomic_load(&a.ctr->long_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (nx >= ct || nx >= a.long_cap) break;
+          if (atomicCAS(&a.ctr->long_next, nx, nx + 1u) == nx) {
+            k = nx;
+            break;
+          }
+        }
+      }
+      k = __shfl(k, 0, 64);
+      if (k == ~0u) break;
+      uint32_t ls = 0, lrec = 0, lt = 0;
+      if (lane == 0) {
+        // the adding lane stores the last word right after taking the index
+        while ((lt = __hip_atomic_load(&a.long_q[3 * k + 2], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) == 0) {
+        }
+        ls = a.long_q[3 * k];
+        lrec = a.long_q[3 * k + 1];
+        a.long_q[3 * k + 2] = 0;  // clean for the next launch
+      }
+      ls = __shfl(ls, 0, 64);
+      lrec = __shfl(lrec, 0, 64);
+      lt = __shfl(lt, 0, 64);
+      const Span sp = a.spans[span_of(a.spans, a.n_spans, ls)];
+      WinReader rd(sp.base - sp.vbase, sp.vend, ar, wbytes);
+      LaneAcc D;
+      acc_zero(D);
+      process_line<NS, GRP, LB>(rd, a, P, ls, lrec, lt - 1u, D, lgp, lane == 0);
+      if (lane == 0) acc_merge(L, D);
+      __syncthreads();
+    }
   }
 
   // ---- flush ----
@@ -1107,12 +1317,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   if (GRP) {
     __syncthreads();
     LGroup &lg = *lgp;
-    DevReader gr(a.d, a.nq);
     for (uint32_t e = lane; e < LGROUP; e += 64) {
       unsigned long long k = lg.key[e];
       if (k == EMPTY_KEY) continue;
       GKey gk = unpack_key(k);
-      uint32_t pos = group_find(gr, gr, a.gt, gk, lg.hash[e]);
+      DevReader gr = span_reader(a.spans, a.n_spans, gk.off);
+      uint32_t pos = group_find(gr, a.spans, a.n_spans, a.gt, gk, lg.hash[e]);
       if (pos == ~0u) {
         atomicOr(&a.ctr->group_overflow, 1u);
         continue;
@@ -1133,7 +1343,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 
 // One lane per selected row: re-evaluate and write its bytes.
 template <int NS>
-__global__ __launch_bounds__(256) void k_emit(const uint8_t *d, uint32_t nq, DProg P0, const uint8_t *tab,
+__global__ __launch_bounds__(256) void k_emit(const Span *spans, uint32_t n_spans, DProg P0, const uint8_t *tab,
                                                uint32_t tab_bytes, const DevCtr *ctr, const uint32_t *sel_idx,
                                                const uint64_t *row_off, const uint32_t *sel_start, uint8_t *out) {
   const LProg P = stage_prog(P0, tab, tab_bytes, (LDS_AS uint8_t *)wjq_dyn_lds);
@@ -1143,7 +1353,7 @@ __global__ __launch_bounds__(256) void k_emit(const uint8_t *d, uint32_t nq, DPr
        r += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t idx = sel_idx[r];
     uint32_t s = sel_start[idx];
-    DevReader rd(d, nq);
+    DevReader rd = span_reader(spans, n_spans, s);
     SlotV sl[NS];
     LineResult lr = eval_line<NS>(rd, s, P, sl);
     DevSink w{out, row_off[r]};
@@ -1159,7 +1369,7 @@ __global__ __launch_bounds__(256) void k_emit(const uint8_t *d, uint32_t nq, DPr
 #define WJQ_DECLARE_LAUNCHERS(NS)                                                                        \
   void launch_main_##NS(bool grouped, bool lb, uint32_t grid, hipStream_t st, const KArgs &a);          \
   int occupancy_main_##NS(bool grouped, bool lb, uint32_t lds);                                         \
-  void launch_emit_##NS(uint32_t grid, hipStream_t st, const uint8_t *d, uint32_t n, const DProg &P,    \
+  void launch_emit_##NS(uint32_t grid, hipStream_t st, const Span *spans, uint32_t n_spans, const DProg &P, \
                         const uint8_t *tab, uint32_t tab_bytes, const DevCtr *ctr, const uint32_t *sel_idx, \
                         const uint64_t *row_off, const uint32_t *sel_start, uint8_t *out);
 WJQ_DECLARE_LAUNCHERS(1)
@@ -1183,9 +1393,9 @@ WJQ_DECLARE_LAUNCHERS(16)
     else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_main<NS, false, false>, 64, lds);         \
     return e == hipSuccess ? b : 0;                                                                       \
   }                                                                                                       \
-  void launch_emit_##NS(uint32_t grid, hipStream_t st, const uint8_t *d, uint32_t n, const DProg &P,     \
+  void launch_emit_##NS(uint32_t grid, hipStream_t st, const Span *spans, uint32_t n_spans, const DProg &P, \
                         const uint8_t *tab, uint32_t tab_bytes, const DevCtr *ctr, const uint32_t *sel_idx, \
                         const uint64_t *row_off, const uint32_t *sel_start, uint8_t *out) {              \
-    hipLaunchKernelGGL((k_emit<NS>), dim3(grid), dim3(256), (tab_bytes + 15u) & ~15u, st, d, n, P, tab,     \
+    hipLaunchKernelGGL((k_emit<NS>), dim3(grid), dim3(256), (tab_bytes + 15u) & ~15u, st, spans, n_spans, P, tab, \
                        tab_bytes, ctr, sel_idx, row_off, sel_start, out);                                \
   }

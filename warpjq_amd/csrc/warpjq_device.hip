@@ -18,6 +18,8 @@ __global__ void k_reset(DevCtr *ctr, unsigned long long *tile_state, uint32_t nt
     ctr->overflow = ctr->saw_non_numeric = ctr->group_overflow = 0;
     ctr->fb_count = ctr->n_groups = ctr->ticket = ctr->tiles_done = 0;
     for (int r = 0; r < 8; r++) ctr->rticket[r] = 0;
+    ctr->blob_used = 0;
+    ctr->long_count = ctr->long_next = 0;
   }
   for (uint32_t t = i; t < ntiles; t += gridDim.x * blockDim.x) tile_state[t] = 0;
 }
@@ -107,14 +109,31 @@ __global__ __launch_bounds__(256) void k_agg_final(const AggPart *parts, uint32_
   }
 }
 
-__global__ void k_group_compact(GroupTab g, DevCtr *ctr, warpjq_group *out, warpjq_amd_agg_ext *ext, uint32_t mis) {
+// Key offsets: relative to the chunk's first byte (the ABI's meaning), or,
+// for a batched submit, keys are copied into `blob` and key_off points there
+// (there is no single buffer the caller could index).
+__global__ void k_group_compact(GroupTab g, DevCtr *ctr, warpjq_group *out, warpjq_amd_agg_ext *ext, const Span *spans,
+                                uint32_t n_spans, uint8_t *blob, uint32_t blob_cap) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.cap; i += gridDim.x * blockDim.x) {
     unsigned long long k = g.keys[i];
     if (k == EMPTY_KEY) continue;
     GKey gk = unpack_key(k);
     uint32_t o = atomicAdd(&ctr->n_groups, 1u);
     warpjq_group r;
-    r.key_off = gk.kind == JK_NULL ? 0 : gk.off - mis;  // q position -> chunk offset
+    if (gk.kind == JK_NULL) {
+      r.key_off = 0;
+    } else if (!blob) {
+      r.key_off = gk.off - spans[0].start;
+    } else {
+      const uint32_t at = atomicAdd(&ctr->blob_used, gk.len);
+      if ((uint64_t)at + gk.len <= blob_cap) {
+        DevReader rd = span_reader(spans, n_spans, gk.off);
+        for (uint32_t q = 0; q < gk.len; q++) blob[at + q] = (uint8_t)rd.byte(gk.off + q);
+      } else {
+        atomicOr(&ctr->group_overflow, 1u);
+      }
+      r.key_off = at;
+    }
     r.key_len = gk.len;
     r.key_kind = gk.kind;
     r._pad = 0;
@@ -371,7 +390,15 @@ struct Slot {
 
   uint32_t *d_fb_idx = nullptr, *d_fb_off = nullptr, *d_fb_len = nullptr, *d_fb_tile = nullptr;
   uint32_t ntiles = 0;
-  uint32_t *h_fb_idx = nullptr, *h_fb_off = nullptr, *h_fb_len = nullptr;
+  uint32_t *h_fb_idx = nullptr, *h_fb_off = nullptr, *h_fb_len = nullptr, *h_fb_span = nullptr;
+
+  // the submit's spans (device copy, pinned staging for it, host view)
+  Span *d_spans = nullptr, *h_spans = nullptr;
+  warpjq_amd_span *h_pieces = nullptr;
+  uint32_t *d_long = nullptr;  // deferred long lines, 3 words each, zero between launches
+  bool keys_blob = false;  // group keys copied out (batched submits)
+  uint32_t n_spans = 0;
+  uint8_t *d_keyblob = nullptr, *h_keyblob = nullptr;
 
   bool submitted = false;
 };
@@ -464,6 +491,13 @@ static void free_slot(Slot &s) {
   (void)hipFree(s.d_fb_off);
   (void)hipFree(s.d_fb_len);
   (void)hipFree(s.d_fb_tile);
+  (void)hipFree(s.d_spans);
+  (void)hipFree(s.d_long);
+  (void)hipHostFree(s.h_spans);
+  delete[] s.h_pieces;
+  (void)hipHostFree(s.h_fb_span);
+  (void)hipFree(s.d_keyblob);
+  (void)hipHostFree(s.h_keyblob);
   (void)hipHostFree(s.h_fb_idx);
   (void)hipHostFree(s.h_fb_off);
   (void)hipHostFree(s.h_fb_len);
@@ -497,6 +531,13 @@ static warpjq_status alloc_slot(warpjq_ctx *ctx, Slot &s, char *err, size_t err_
   HIP_TRY(dalloc(&s.d_fb_off, fc), "hipMalloc(fb)");
   HIP_TRY(dalloc(&s.d_fb_len, fc), "hipMalloc(fb)");
   HIP_TRY(dalloc(&s.d_fb_tile, fc), "hipMalloc(fb)");
+  HIP_TRY(halloc(&s.h_fb_span, fc), "hipHostMalloc(fb)");
+  HIP_TRY(dalloc(&s.d_spans, MAX_SPANS), "hipMalloc(spans)");
+  HIP_TRY(dalloc(&s.d_long, (size_t)ctx->max_tiles * 3), "hipMalloc(long lines)");
+  HIP_TRY(hipMemset(s.d_long, 0, (size_t)ctx->max_tiles * 12), "hipMemset(long lines)");
+  HIP_TRY(halloc(&s.h_spans, MAX_SPANS), "hipHostMalloc(spans)");
+  s.h_pieces = new (std::nothrow) warpjq_amd_span[MAX_SPANS];
+  if (!s.h_pieces) return WARPJQ_ERR_OOM;
   HIP_TRY(halloc(&s.h_fb_idx, fc), "hipHostMalloc(fb)");
   HIP_TRY(halloc(&s.h_fb_off, fc), "hipHostMalloc(fb)");
   HIP_TRY(halloc(&s.h_fb_len, fc), "hipHostMalloc(fb)");
@@ -517,6 +558,8 @@ static warpjq_status alloc_slot(warpjq_ctx *ctx, Slot &s, char *err, size_t err_
     HIP_TRY(halloc(&s.h_groups, G), "hipHostMalloc(groups)");
     HIP_TRY(dalloc(&s.d_gext, G), "hipMalloc(groups)");
     HIP_TRY(halloc(&s.h_gext, G), "hipHostMalloc(groups)");
+    HIP_TRY(dalloc(&s.d_keyblob, KEYBLOB_BYTES), "hipMalloc(keys)");
+    HIP_TRY(halloc(&s.h_keyblob, KEYBLOB_BYTES), "hipHostMalloc(keys)");
     hipLaunchKernelGGL(k_group_init, dim3(1024), dim3(256), 0, s.stream, s.gt);
     HIP_TRY(hipGetLastError(), "k_group_init");
   }
@@ -603,6 +646,46 @@ warpjq_status warpjq_abi_check(const uint64_t *sizes, size_t n) {
   return WARPJQ_OK;
 }
 
+// Uploads the program tables (one allocation of TAB_LDS_MAX bytes, kept for
+// the context's life) and points ctx->dprog at them; sets the LDS table
+// size and the grid that follows from it. ctx->host must hold the program.
+static warpjq_status put_program(warpjq_ctx *ctx, const warpjq_program *prog, char *err, size_t err_len) {
+  std::vector<uint8_t> buf;
+  const DProg &h = ctx->host.d;
+  size_t o_steps = put_table(buf, prog->steps, prog->n_steps);
+  size_t o_cmps = put_table(buf, prog->cmps, prog->n_cmps);
+  size_t o_cond = put_table(buf, prog->cond_rpn, prog->n_cond);
+  size_t o_blob = put_table(buf, prog->blob, prog->n_blob);
+  size_t o_spath = put_table(buf, ctx->host.spath.data(), ctx->host.spath.size());
+  size_t o_sop = put_table(buf, prog->slot_of_path, prog->n_paths);
+  size_t o_fs = put_table(buf, ctx->host.field_slot.data(), ctx->host.field_slot.size());
+  size_t o_po = put_table(buf, prog->prefix_off, prog->n_fields);
+  size_t o_pl = put_table(buf, prog->prefix_len, prog->n_fields);
+  buf.resize((buf.size() + 15) & ~(size_t)15);
+  if (buf.size() > TAB_LDS_MAX) {
+    set_err(err, err_len, "query is too large for the kernels", "its tables exceed the 16 KiB staged in LDS");
+    return WARPJQ_ERR_INVALID_ARG;
+  }
+  if (!ctx->d_tables) HIP_TRY(hipMalloc(&ctx->d_tables, TAB_LDS_MAX), "uploading the compiled query");
+  if (!buf.empty())
+    HIP_TRY(hipMemcpy(ctx->d_tables, buf.data(), buf.size(), hipMemcpyHostToDevice), "uploading the compiled query");
+  ctx->tab_bytes = (uint32_t)buf.size();
+  ctx->grid = main_grid(ctx);
+  uint8_t *base = (uint8_t *)ctx->d_tables;
+  DProg d = h;
+  d.steps = (const warpjq_step *)(base + o_steps);
+  d.cmps = (const warpjq_cmp *)(base + o_cmps);
+  d.cond = (const warpjq_cond_op *)(base + o_cond);
+  d.blob = base + o_blob;
+  d.spath = (const SlotPath *)(base + o_spath);
+  d.slot_of_path = (const uint32_t *)(base + o_sop);
+  d.field_slot = (const uint32_t *)(base + o_fs);
+  d.prefix_off = (const uint32_t *)(base + o_po);
+  d.prefix_len = (const uint32_t *)(base + o_pl);
+  ctx->dprog = d;
+  return WARPJQ_OK;
+}
+
 warpjq_status warpjq_ctx_create(const warpjq_program *prog, uint64_t max_chunk_bytes, uint32_t n_slots,
                                 warpjq_ctx **out, char *err, size_t err_len) {
   if (!prog || !out || n_slots == 0 || max_chunk_bytes == 0) return WARPJQ_ERR_INVALID_ARG;
@@ -637,45 +720,10 @@ warpjq_status warpjq_ctx_create(const warpjq_program *prog, uint64_t max_chunk_b
   // tiles for the largest chunk a slot or a resident buffer may hold
   ctx->max_tiles = (uint32_t)((MAX_CHUNK + 16 + CLAIM_MIN - 1) / CLAIM_MIN) + 1;
 
-  // one device allocation holding every program table
-  std::vector<uint8_t> buf;
-  const DProg &h = ctx->host.d;
-  size_t o_steps = put_table(buf, prog->steps, prog->n_steps);
-  size_t o_cmps = put_table(buf, prog->cmps, prog->n_cmps);
-  size_t o_cond = put_table(buf, prog->cond_rpn, prog->n_cond);
-  size_t o_blob = put_table(buf, prog->blob, prog->n_blob);
-  size_t o_spath = put_table(buf, ctx->host.spath.data(), ctx->host.spath.size());
-  size_t o_sop = put_table(buf, prog->slot_of_path, prog->n_paths);
-  size_t o_fs = put_table(buf, ctx->host.field_slot.data(), ctx->host.field_slot.size());
-  size_t o_po = put_table(buf, prog->prefix_off, prog->n_fields);
-  size_t o_pl = put_table(buf, prog->prefix_len, prog->n_fields);
-  buf.resize((buf.size() + 15) & ~(size_t)15);
-  if (buf.size() > TAB_LDS_MAX) {
-    set_err(err, err_len, "query is too large for the kernels", "its tables exceed the 16 KiB staged in LDS");
+  if (warpjq_status st = put_program(ctx, prog, err, err_len)) {
     warpjq_ctx_destroy(ctx);
-    return WARPJQ_ERR_INVALID_ARG;
+    return st;
   }
-  ctx->tab_bytes = (uint32_t)buf.size();
-  ctx->grid = main_grid(ctx);
-  hipError_t e = hipMalloc(&ctx->d_tables, buf.size());
-  if (e == hipSuccess) e = hipMemcpy(ctx->d_tables, buf.data(), buf.size(), hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    set_err(err, err_len, "uploading the compiled query", hipGetErrorString(e));
-    warpjq_ctx_destroy(ctx);
-    return WARPJQ_ERR_CUDA;
-  }
-  uint8_t *base = (uint8_t *)ctx->d_tables;
-  DProg d = h;
-  d.steps = (const warpjq_step *)(base + o_steps);
-  d.cmps = (const warpjq_cmp *)(base + o_cmps);
-  d.cond = (const warpjq_cond_op *)(base + o_cond);
-  d.blob = base + o_blob;
-  d.spath = (const SlotPath *)(base + o_spath);
-  d.slot_of_path = (const uint32_t *)(base + o_sop);
-  d.field_slot = (const uint32_t *)(base + o_fs);
-  d.prefix_off = (const uint32_t *)(base + o_po);
-  d.prefix_len = (const uint32_t *)(base + o_pl);
-  ctx->dprog = d;
 
   ctx->slots = new (std::nothrow) Slot[n_slots];
   if (!ctx->slots) {
@@ -703,6 +751,28 @@ void warpjq_ctx_destroy(warpjq_ctx *ctx) {
   delete ctx;
 }
 
+warpjq_status warpjq_amd_ctx_reprogram(warpjq_ctx *ctx, const warpjq_program *prog, char *err, size_t err_len) {
+  if (!ctx || !prog) return WARPJQ_ERR_INVALID_ARG;
+  const bool grouped = prog->output_kind == WARPJQ_OUT_AGG && prog->group_path != 0xFFFFFFFFu;
+  const bool rows = prog->output_kind != WARPJQ_OUT_AGG;
+  if (grouped != ctx->grouped || rows != ctx->rows) {
+    set_err(err, err_len, "reprogram", "the new query needs other buffers (grouped / row output differ)");
+    return WARPJQ_ERR_INVALID_ARG;
+  }
+  DProgHost host;
+  std::string why;
+  if (!build_dprog(*prog, host, why)) {
+    set_err(err, err_len, "query is not supported by the kernels", why.c_str());
+    return WARPJQ_ERR_INVALID_ARG;
+  }
+  for (uint32_t i = 0; i < ctx->n_slots; i++) HIP_TRY(hipStreamSynchronize(ctx->slots[i].stream), "reprogram");
+  DProgHost old = ctx->host;
+  ctx->host = host;
+  warpjq_status st = put_program(ctx, prog, err, err_len);
+  if (st != WARPJQ_OK) ctx->host = old;  // tables too large: the old program stays
+  return st;
+}
+
 uint8_t *warpjq_slot_buffer(warpjq_ctx *ctx, uint32_t slot) {
   if (!ctx || slot >= ctx->n_slots) return nullptr;
   return ctx->slots[slot].h_pinned;
@@ -714,23 +784,40 @@ uint64_t warpjq_max_lines(const warpjq_ctx *ctx) { return ctx ? ctx->max_lines :
 
 }  // extern "C"
 
-// Queues the kernel chain for `n` bytes at `src` (device memory) on `s`.
-static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const uint8_t *src, uint64_t n, char *err,
+// Queues the kernel chain over the device spans `in[0..n_in)` on `s`.
+static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const warpjq_amd_span *in, uint32_t n_in, char *err,
                                   size_t err_len) {
   hipStream_t st = s.stream;
   const uint32_t claim = ctx->claim;
-  const uint32_t mis = (uint32_t)((uintptr_t)src & 15u);
-  const uint8_t *base = src - mis;  // kernels address the chunk from its aligned-down base
-  const uint64_t nq = n + mis;
-  const uint32_t ntiles = n ? (uint32_t)((nq + claim - 1) / claim) : 0;
+  // place the spans in the kernels' position space, each on a claim boundary
+  uint64_t v = 0, total = 0;
+  uint32_t ns = 0;
+  for (uint32_t i = 0; i < n_in; i++) {
+    if (in[i].n_bytes == 0) continue;
+    const uint32_t mis = (uint32_t)((uintptr_t)in[i].d_bytes & 15u);
+    Span sp;
+    sp.base = in[i].d_bytes - mis;
+    sp.vbase = (uint32_t)v;
+    sp.start = (uint32_t)(v + mis);
+    sp.vend = (uint32_t)(v + mis + in[i].n_bytes);
+    sp._pad = 0;
+    s.h_spans[ns++] = sp;
+    total += in[i].n_bytes;
+    v = ((v + mis + in[i].n_bytes + claim - 1) / claim) * claim;
+  }
+  if (ns == 0) {  // keep one empty span so results can be read uniformly
+    s.h_spans[0] = Span{in && n_in ? in[0].d_bytes : nullptr, 0, 0, 0, 0};
+  }
+  s.n_spans = ns ? ns : 1;
+  const uint32_t ntiles = (uint32_t)(v / claim);
   s.ntiles = ntiles;
   hipLaunchKernelGGL(k_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (ntiles + 255) / 256))), dim3(256),
                      0, st, s.d_ctr, s.d_tile_state, ntiles);
-  if (n > 0) {
+  if (total > 0) {
+    HIP_TRY(hipMemcpyAsync(s.d_spans, s.h_spans, (size_t)ns * sizeof(Span), hipMemcpyHostToDevice, st), "H2D(spans)");
     KArgs a{};
-    a.d = base;
-    a.mis = mis;
-    a.nq = (uint32_t)nq;
+    a.spans = s.d_spans;
+    a.n_spans = ns;
     a.ntiles = ntiles;
     a.claim = claim;
     a.arena = ctx->arena;
@@ -749,6 +836,8 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const uint8_t *src, 
     a.fb_len = s.d_fb_len;
     a.fb_tile = s.d_fb_tile;
     a.fb_cap = ctx->fb_cap;
+    a.long_q = s.d_long;
+    a.long_cap = ctx->max_tiles;
     uint32_t grid = std::min<uint32_t>(ctx->grid, ntiles);
     HIP_TRY(hipEventRecord(s.evm0, st), "hipEventRecord");
     switch (ctx->host.ns) {
@@ -762,12 +851,13 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const uint8_t *src, 
     HIP_TRY(hipEventRecord(s.evm1, st), "hipEventRecord");
     if (ctx->dprog.output_kind == WARPJQ_OUT_AGG) {
       if (ctx->grouped) {
-        hipLaunchKernelGGL(k_group_compact, dim3(1024), dim3(256), 0, st, s.gt, s.d_ctr, s.d_groups, s.d_gext, mis);
+        hipLaunchKernelGGL(k_group_compact, dim3(1024), dim3(256), 0, st, s.gt, s.d_ctr, s.d_groups, s.d_gext, s.d_spans,
+                           ns, s.keys_blob ? s.d_keyblob : nullptr, KEYBLOB_BYTES);
       } else {
         hipLaunchKernelGGL(k_agg_final, dim3(1), dim3(256), 0, st, s.d_parts, grid, s.d_ctr, s.d_aggout);
       }
     } else {
-      uint64_t lines_bound = std::min<uint64_t>(ctx->max_lines, n + 1);
+      uint64_t lines_bound = std::min<uint64_t>(ctx->max_lines, total + ns);
       uint32_t nb = (uint32_t)((lines_bound + ROWSCAN_BLOCK - 1) / ROWSCAN_BLOCK);
       hipLaunchKernelGGL(k_row_reduce, dim3(nb), dim3(256), 0, st, s.d_row_len, s.d_ctr, ctx->max_lines, s.d_blk_cnt,
                          s.d_blk_bytes);
@@ -779,11 +869,11 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const uint8_t *src, 
       uint32_t eg = std::min<uint32_t>(ctx->grid, (uint32_t)((lines_bound + 255) / 256));
       if (eg == 0) eg = 1;
       switch (ctx->host.ns) {
-        case 1: launch_emit_1(eg, st, base, (uint32_t)nq, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
-        case 2: launch_emit_2(eg, st, base, (uint32_t)nq, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
-        case 4: launch_emit_4(eg, st, base, (uint32_t)nq, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
-        case 8: launch_emit_8(eg, st, base, (uint32_t)nq, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
-        default: launch_emit_16(eg, st, base, (uint32_t)nq, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        case 1: launch_emit_1(eg, st, s.d_spans, ns, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        case 2: launch_emit_2(eg, st, s.d_spans, ns, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        case 4: launch_emit_4(eg, st, s.d_spans, ns, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        case 8: launch_emit_8(eg, st, s.d_spans, ns, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        default: launch_emit_16(eg, st, s.d_spans, ns, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
       }
     }
     HIP_TRY(hipGetLastError(), "kernel chain");
@@ -808,7 +898,9 @@ warpjq_status warpjq_submit(warpjq_ctx *ctx, uint32_t slot, uint64_t n_bytes, ch
   HIP_TRY(hipEventRecord(s.ev0, s.stream), "hipEventRecord");
   if (n_bytes) HIP_TRY(hipMemcpyAsync(s.d_data, s.h_pinned, n_bytes, hipMemcpyHostToDevice, s.stream), "H2D chunk");
   HIP_TRY(hipEventRecord(s.ev1, s.stream), "hipEventRecord");
-  return launch_chain(ctx, s, s.d_data, n_bytes, err, err_len);
+  const warpjq_amd_span one{s.d_data, n_bytes};
+  s.keys_blob = false;
+  return launch_chain(ctx, s, &one, 1, err, err_len);
 }
 
 warpjq_status warpjq_amd_submit_device(warpjq_ctx *ctx, uint32_t slot, const uint8_t *d_bytes, uint64_t n_bytes,
@@ -821,7 +913,90 @@ warpjq_status warpjq_amd_submit_device(warpjq_ctx *ctx, uint32_t slot, const uin
   s.d_src = d_bytes;
   HIP_TRY(hipEventRecord(s.ev0, s.stream), "hipEventRecord");
   HIP_TRY(hipEventRecord(s.ev1, s.stream), "hipEventRecord");
-  return launch_chain(ctx, s, d_bytes, n_bytes, err, err_len);
+  const warpjq_amd_span one{d_bytes, n_bytes};
+  s.keys_blob = false;
+  return launch_chain(ctx, s, &one, 1, err, err_len);
+}
+
+warpjq_status warpjq_amd_submit_spans(warpjq_ctx *ctx, uint32_t slot, const warpjq_amd_span *spans, uint32_t n_spans,
+                                      char *err, size_t err_len) {
+  if (!ctx || slot >= ctx->n_slots || (!spans && n_spans) || n_spans > MAX_SPANS) return WARPJQ_ERR_INVALID_ARG;
+  uint64_t v = 0, total = 0;
+  for (uint32_t i = 0; i < n_spans; i++) {
+    if (!spans[i].d_bytes && spans[i].n_bytes) return WARPJQ_ERR_INVALID_ARG;
+    if (!spans[i].n_bytes) continue;
+    const uint64_t mis = (uintptr_t)spans[i].d_bytes & 15u;
+    v = ((v + mis + spans[i].n_bytes + ctx->claim - 1) / ctx->claim) * ctx->claim;
+    total += spans[i].n_bytes;
+  }
+  if (v > MAX_CHUNK) {
+    set_err(err, err_len, "batch is too large", "the spans of one submit must fit 4 GiB - 1 MiB of positions");
+    return WARPJQ_ERR_INVALID_ARG;
+  }
+  if (ctx->rows && total > ctx->chunk_cap) {
+    set_err(err, err_len, "batch is too large", "row outputs are bounded by the slot capacity");
+    return WARPJQ_ERR_INVALID_ARG;
+  }
+  Slot &s = ctx->slots[slot];
+  s.n_bytes = total;
+  s.resident = true;
+  s.d_src = n_spans ? spans[0].d_bytes : nullptr;
+  HIP_TRY(hipEventRecord(s.ev0, s.stream), "hipEventRecord");
+  HIP_TRY(hipEventRecord(s.ev1, s.stream), "hipEventRecord");
+  s.keys_blob = true;
+  return launch_chain(ctx, s, spans, n_spans, err, err_len);
+}
+
+warpjq_status warpjq_amd_submit_pieces(warpjq_ctx *ctx, uint32_t slot, uint64_t n_bytes, const warpjq_amd_piece *pieces,
+                                       uint32_t n_pieces, char *err, size_t err_len) {
+  if (!ctx || slot >= ctx->n_slots || n_bytes > ctx->chunk_cap || (!pieces && n_pieces) || n_pieces > MAX_SPANS)
+    return WARPJQ_ERR_INVALID_ARG;
+  Slot &s = ctx->slots[slot];
+  warpjq_amd_span *sp = s.h_pieces;
+  for (uint32_t i = 0; i < n_pieces; i++) {
+    if (pieces[i].off > n_bytes || pieces[i].len > n_bytes - pieces[i].off) {
+      set_err(err, err_len, "submit_pieces", "a piece lies outside the slot's bytes");
+      return WARPJQ_ERR_INVALID_ARG;
+    }
+    sp[i].d_bytes = s.d_data + pieces[i].off;
+    sp[i].n_bytes = pieces[i].len;
+  }
+  s.n_bytes = n_bytes;
+  s.resident = false;
+  s.d_src = s.d_data;
+  HIP_TRY(hipEventRecord(s.ev0, s.stream), "hipEventRecord");
+  if (n_bytes) HIP_TRY(hipMemcpyAsync(s.d_data, s.h_pinned, n_bytes, hipMemcpyHostToDevice, s.stream), "H2D chunk");
+  HIP_TRY(hipEventRecord(s.ev1, s.stream), "hipEventRecord");
+  s.keys_blob = true;
+  return launch_chain(ctx, s, sp, n_pieces, err, err_len);
+}
+
+uint8_t *warpjq_amd_device_upload(const uint8_t *bytes, uint64_t n, char *err, size_t err_len) {
+  if (!bytes && n) return nullptr;
+  uint8_t *p = nullptr;
+  hipError_t e = hipMalloc((void **)&p, n + 256);
+  if (e == hipSuccess && n) e = hipMemcpy(p, bytes, n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(p + n, '\n', 256);
+  if (e != hipSuccess) {
+    set_err(err, err_len, "device_upload", hipGetErrorString(e));
+    if (p) (void)hipFree(p);
+    return nullptr;
+  }
+  return p;
+}
+
+void warpjq_amd_device_free(uint8_t *p) {
+  if (p) (void)hipFree(p);
+}
+
+const uint32_t *warpjq_amd_fallback_spans(const warpjq_ctx *ctx, uint32_t slot) {
+  if (!ctx || slot >= ctx->n_slots) return nullptr;
+  return ctx->slots[slot].h_fb_span;
+}
+
+const uint8_t *warpjq_amd_group_keys(const warpjq_ctx *ctx, uint32_t slot) {
+  if (!ctx || slot >= ctx->n_slots) return nullptr;
+  return ctx->slots[slot].h_keyblob;
 }
 
 warpjq_status warpjq_wait(warpjq_ctx *ctx, uint32_t slot, warpjq_chunk_result *out, char *err, size_t err_len) {
@@ -860,6 +1035,19 @@ warpjq_status warpjq_wait(warpjq_ctx *ctx, uint32_t slot, warpjq_chunk_result *o
     HIP_TRY(hipMemcpyAsync(s.h_fb_idx, s.d_fb_idx, nfb * 4ull, hipMemcpyDeviceToHost, s.stream), "D2H(fallback)");
     HIP_TRY(hipMemcpyAsync(s.h_fb_off, s.d_fb_off, nfb * 4ull, hipMemcpyDeviceToHost, s.stream), "D2H(fallback)");
     HIP_TRY(hipMemcpyAsync(s.h_fb_len, s.d_fb_len, nfb * 4ull, hipMemcpyDeviceToHost, s.stream), "D2H(fallback)");
+    HIP_TRY(hipStreamSynchronize(s.stream), "sync(fallback)");
+    // q positions -> (span, offset from the span's first byte)
+    for (uint32_t k = 0; k < nfb; k++) {
+      const uint32_t q = s.h_fb_off[k];
+      uint32_t lo = 0, hi = s.n_spans;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (s.h_spans[mid].vbase <= q) lo = mid;
+        else hi = mid;
+      }
+      s.h_fb_span[k] = lo;
+      s.h_fb_off[k] = q - s.h_spans[lo].start;
+    }
   }
   out->fallback_idx = s.h_fb_idx;
   out->fallback_off = s.h_fb_off;
@@ -868,6 +1056,10 @@ warpjq_status warpjq_wait(warpjq_ctx *ctx, uint32_t slot, warpjq_chunk_result *o
   if (ctx->dprog.output_kind == WARPJQ_OUT_AGG) {
     if (ctx->grouped) {
       uint32_t ng = std::min<uint32_t>(c.n_groups, ctx->group_cap);
+      if (s.keys_blob && c.blob_used)
+        HIP_TRY(hipMemcpyAsync(s.h_keyblob, s.d_keyblob, std::min<uint32_t>(c.blob_used, KEYBLOB_BYTES),
+                               hipMemcpyDeviceToHost, s.stream),
+                "D2H(group keys)");
       if (ng) {
         HIP_TRY(hipMemcpyAsync(s.h_groups, s.d_groups, (size_t)ng * sizeof(warpjq_group), hipMemcpyDeviceToHost,
                                s.stream),
