@@ -14,6 +14,13 @@ lognormal sizes clamped to [4 KB, 256 MB], 16 GiB in all, 1% of them holding
 a line of up to 1 MB, resident in HBM; the filter-count over all of them
 once per file (one launch each, two slots in flight) and batched (spans of
 many files per launch, warpjq_amd_submit_spans).
+
+`--workload served` measures the served form: 64 nginx datasets of
+log-uniform sizes in [1 MB, 2 GB] kept resident by a warpjq_amd server
+(200 GB budget, 4 workers), requests from 12 query templates with random
+constants arriving as a Poisson process at a swept rate; reports p50/p99
+latency (arrival to result) and throughput per rate, and checks that a
+sample of requests gives the same bytes alone as in the mix.
 """
 import argparse
 import json
@@ -179,6 +186,119 @@ def bench_files(args, w, torch, np):
     print(json.dumps(line), flush=True)
 
 
+SERVED_HOSTS = ["web-01.prod.us-east-1.example.com", "api-1.prod.eu-west-2.internal",
+                "cdn-edge.ap-south-1.example.net", "auth.prod.us-east-1.example.com"]
+SERVED_PATHS = ["/", "/api/v1/users", "/search", "/checkout", "/health", "/static/app.js", "/admin"]
+
+
+def served_query(rng):
+    """One of 12 templates with random constants."""
+    s = int(rng.choice([200, 404, 500, 301, 503]))
+    b = int(rng.integers(100, 100000))
+    h = SERVED_HOSTS[int(rng.integers(len(SERVED_HOSTS)))]
+    m = ["GET", "POST", "PUT", "DELETE", "HEAD"][int(rng.integers(5))]
+    p = SERVED_PATHS[int(rng.integers(len(SERVED_PATHS)))]
+    t = [f"select(.status == {s}) | count", f"select(.bytes > {b}) | count", "sum(.bytes)", "avg(.bytes)",
+         "max(.bytes)", "group_by(.host) | count", "group_by(.status) | sum(.bytes)",
+         f"select(.status == {s}) | group_by(.path) | count", f'select(.method == "{m}") | count',
+         f"select(.status >= {s} and .bytes > {b}) | count",
+         f'select(.status == 503 and .host == "{h}") | {{p: .path, b: .bytes}}',
+         f'select(.path == "{p}") | count']
+    return t[int(rng.integers(len(t)))]
+
+
+def bench_served(args, w, np):
+    import threading
+    n_ds = args.datasets
+    rng = np.random.default_rng(args.seed)
+    sizes = np.exp(rng.uniform(np.log(1e6), np.log(2e9), size=n_ds)).astype(np.int64)
+    total = int(sizes.sum())
+    t0 = time.time()
+    host, n, n_lines, hist = w.gen_nginx(total, seed=args.seed, threads=min(16, os.cpu_count() or 1))
+    log(f"generated {n} bytes in {time.time() - t0:.1f}s")
+    bounds, a = [], 0
+    for sz in sizes:
+        b = min(n, a + int(sz))
+        while b < n and host[b - 1] != 10:
+            b += 1
+        if b > a:
+            bounds.append((a, b))
+        a = b
+    srv = w.Server(hbm_budget=200 << 30, workers=args.workers)
+    ids = [srv.add_dataset(host[a:b]) for a, b in bounds]
+    t0 = time.time()
+    for h in [srv.submit("count", [i]) for i in ids]:  # load every dataset once
+        srv.wait(h)
+    log(f"{len(ids)} datasets resident in {time.time() - t0:.1f}s")
+    sizes_b = [b - a for a, b in bounds]
+    results = []
+    for rate in args.rates:
+        reqs = []
+        for _ in range(args.requests):
+            reqs.append((served_query(rng), int(rng.integers(len(ids)))))
+        gaps = rng.exponential(1.0 / rate, size=len(reqs))
+        handles = [None] * len(reqs)
+        done = [None] * len(reqs)
+
+        def collect():
+            for k in range(len(reqs)):
+                while handles[k] is None:
+                    time.sleep(0.0002)
+                out, st, lat = srv.wait(handles[k])
+                done[k] = (out, lat)
+
+        th = threading.Thread(target=collect)
+        th.start()
+        t_start = time.perf_counter()
+        nxt = t_start
+        for k, (q, d) in enumerate(reqs):
+            nxt += gaps[k]
+            dt = nxt - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            handles[k] = srv.submit(q, [d])
+        th.join()
+        wall = time.perf_counter() - t_start
+        lats = np.array([x[1] for x in done])
+        nbytes = sum(sizes_b[d] for _, d in reqs)
+        # the same request alone, on an idle server: identical bytes
+        sample = rng.choice(len(reqs), size=min(24, len(reqs)), replace=False)
+        for k in sample:
+            q, d = reqs[k]
+            alone, _, _ = srv.wait(srv.submit(q, [d]))
+            if alone != done[k][0]:
+                raise SystemExit(f"served result differs alone vs in the mix: {q} on dataset {d}")
+        r = {"rate_per_s": rate, "requests": len(reqs), "p50_ms": round(float(np.percentile(lats, 50)), 3),
+             "p99_ms": round(float(np.percentile(lats, 99)), 3), "max_ms": round(float(lats.max()), 3),
+             "achieved_req_per_s": round(len(reqs) / wall, 1), "gbps": round(nbytes / wall / 1e9, 1)}
+        results.append(r)
+        log("served", r)
+    st = srv.stats()
+    srv.close()
+    best = max(results, key=lambda r: r["gbps"])
+    line = {
+        "metric": "input GB/s of a Poisson mix of 12 query templates over 64 resident nginx datasets (served form)",
+        "value": best["gbps"],
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.requests,
+        "warmup": len(ids),
+        "ms_per_step": best["p50_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": f"{len(ids)} nginx datasets, log-uniform sizes in [1 MB, 2 GB], resident in HBM, "
+                               f"{args.workers} workers, Poisson arrivals",
+                   "bytes_resident": int(sum(sizes_b)), "budget": 200 << 30, "loads": st["loads"],
+                   "evictions": st["evictions"], "engines_opened": st["engines_opened"],
+                   "engines_reprogrammed": st["engines_reprogrammed"]},
+        "rates": results,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,7 +307,11 @@ def main():
     ap.add_argument("--size-gib", type=float, default=8.0)
     ap.add_argument("--piece-mib", type=int, default=4095)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--workload", choices=["nginx", "files"], default="nginx")
+    ap.add_argument("--workload", choices=["nginx", "files", "served"], default="nginx")
+    ap.add_argument("--datasets", type=int, default=64)
+    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--requests", type=int, default=600)
+    ap.add_argument("--rates", type=float, nargs="+", default=[100.0, 500.0, 2000.0, 5000.0])
     ap.add_argument("--files", type=int, default=4096)
     ap.add_argument("--long-frac", type=float, default=0.01, help="files mix: share of files with a long line")
     ap.add_argument("--no-extras", action="store_true", help="only the timed filter-count loop")
@@ -211,6 +335,8 @@ def main():
     dev = w.device_name()
     if args.workload == "files":
         return bench_files(args, w, torch, np)
+    if args.workload == "served":
+        return bench_served(args, w, np)
     target = int(args.size_gib * (1 << 30))
     t0 = time.time()
     host, n, n_lines, hist = w.gen_nginx(target, seed=args.seed, threads=min(16, os.cpu_count() or 1))

@@ -168,6 +168,7 @@ typedef struct warpjq_amd_server warpjq_amd_server;
 typedef struct {
   uint64_t budget, reserved, loads, evictions, completed, resident;
   double p50_ms, p99_ms, max_ms; /* arrival to result, completed requests */
+  uint64_t engines_opened, engines_reprogrammed; /* contexts created / reused for another query */
 } warpjq_amd_server_stats;
 
 warpjq_amd_server *warpjq_amd_server_create(uint64_t hbm_budget, uint32_t workers, uint64_t chunk_bytes, char *err,

@@ -85,7 +85,8 @@ class Span(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("budget", u64), ("reserved", u64), ("loads", u64), ("evictions", u64), ("completed", u64),
-                ("resident", u64), ("p50_ms", f64), ("p99_ms", f64), ("max_ms", f64)]
+                ("resident", u64), ("p50_ms", f64), ("p99_ms", f64), ("max_ms", f64), ("engines_opened", u64),
+                ("engines_reprogrammed", u64)]
 
 
 class Stats(ctypes.Structure):

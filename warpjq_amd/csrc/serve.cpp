@@ -75,7 +75,7 @@ struct warpjq_amd_server {
   std::deque<Request *> waiting;
   std::map<int64_t, std::unique_ptr<Request>> reqs;
   int64_t next_id = 1;
-  uint64_t tick = 0, loads = 0, evictions = 0, completed = 0;
+  uint64_t tick = 0, loads = 0, evictions = 0, completed = 0, opened = 0, reprogrammed = 0;
   std::vector<double> lat_ms;
   bool stop = false;
   std::vector<std::thread> workers;
@@ -210,15 +210,25 @@ void worker(warpjq_amd_server *s) {
     const std::string key = std::to_string(r->fmt) + ":" + r->query;
     auto it = std::find_if(engines.begin(), engines.end(), [&](const EngineSlot &x) { return x.key == key; });
     char err[1024] = {0};
-    if (it == engines.end() && engines.size() >= ENGINES_PER_WORKER) {
-      // reuse the least recently used engine whose buffers fit this query
-      auto old = std::min_element(engines.begin(), engines.end(),
-                                  [](const EngineSlot &a, const EngineSlot &b) { return a.last < b.last; });
-      int rc = wjqd::engine_reprogram(old->eng, r->query.c_str(), r->fmt, err, sizeof err);
-      if (rc == 0) {
-        old->key = key;
-        it = old;
-      } else if (rc != 2) {
+    if (it == engines.end()) {
+      // a miss: point the least recently used engine whose buffers fit this
+      // query at it (a table upload) rather than opening a context
+      std::vector<std::list<EngineSlot>::iterator> order;
+      for (auto e = engines.begin(); e != engines.end(); ++e) order.push_back(e);
+      std::sort(order.begin(), order.end(), [](auto x, auto y) { return x->last < y->last; });
+      int rc = 2;
+      for (auto e : order) {
+        rc = wjqd::engine_reprogram(e->eng, r->query.c_str(), r->fmt, err, sizeof err);
+        if (rc == 0) {
+          e->key = key;
+          it = e;
+          std::lock_guard<std::mutex> g(s->mu);
+          s->reprogrammed++;
+          break;
+        }
+        if (rc != 2) break;
+      }
+      if (rc != 0 && rc != 2) {
         r->rc = rc;
         r->err = err;
         finish(*s, r);
@@ -243,6 +253,8 @@ void worker(warpjq_amd_server *s) {
         continue;
       }
       it = std::prev(engines.end());
+      std::lock_guard<std::mutex> g(s->mu);
+      s->opened++;
     }
     it->last = ++uses;
     std::vector<const uint8_t *> hp, dp;
@@ -389,6 +401,8 @@ void warpjq_amd_server_get_stats(warpjq_amd_server *s, warpjq_amd_server_stats *
   o->loads = s->loads;
   o->evictions = s->evictions;
   o->completed = s->completed;
+  o->engines_opened = s->opened;
+  o->engines_reprogrammed = s->reprogrammed;
   for (const Dataset &d : s->ds)
     if (d.dev) o->resident++;
   if (!s->lat_ms.empty()) {
