@@ -143,8 +143,27 @@ def bench_files(args, w, torch, np):
         launch_ms.append(round(ctx.timing(pend)[2], 3))
         return count
 
+    gctx = None
+    if args.graphs:
+        # the per-file form again, each file's chain captured once and replayed
+        gctx = w.Context(QUERY, chunk_bytes=64 << 20, n_slots=2)
+        gctx.set_graphs(len(bounds))
+
+    def per_file_graphs():
+        count, pend = 0, None
+        for i, (a, b) in enumerate(bounds):
+            gctx.submit_device(i & 1, base + a, b - a)
+            if pend is not None:
+                count += gctx.wait(pend).agg.count
+            pend = i & 1
+        count += gctx.wait(pend).agg.count
+        return count
+
     res = {}
-    for name, fn in (("per_file", per_file), ("batched", batched)):
+    forms = [("per_file", per_file), ("batched", batched)]
+    if gctx is not None:
+        forms.append(("per_file_graphs", per_file_graphs))
+    for name, fn in forms:
         for _ in range(args.warmup):
             fn()
         torch.cuda.synchronize()
@@ -182,6 +201,7 @@ def bench_files(args, w, torch, np):
         "batched": res["batched"],
         "speedup_batched_vs_per_file": round(speedup, 2),
         "batched_main_ms_per_launch": launch_ms,
+        "per_file_graphs": res.get("per_file_graphs"),
     }
     print(json.dumps(line), flush=True)
 
@@ -314,6 +334,7 @@ def main():
     ap.add_argument("--rates", type=float, nargs="+", default=[100.0, 500.0, 2000.0, 5000.0])
     ap.add_argument("--files", type=int, default=4096)
     ap.add_argument("--long-frac", type=float, default=0.01, help="files mix: share of files with a long line")
+    ap.add_argument("--graphs", action="store_true", help="files mix: also the per-file form with hipGraph replay")
     ap.add_argument("--no-extras", action="store_true", help="only the timed filter-count loop")
     ap.add_argument("--cpu-sample-mib", type=int, default=1024)
     ap.add_argument("--claim-kib", type=int, default=0, help="bytes per wave claim (tuning)")

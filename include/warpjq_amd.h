@@ -119,6 +119,14 @@ warpjq_status warpjq_amd_set_claim(warpjq_ctx *ctx, uint32_t claim_bytes);
  * No counterpart in the reference. */
 warpjq_status warpjq_amd_set_arena(warpjq_ctx *ctx, uint32_t arena_bytes);
 
+/* Keep up to `per_slot` captured kernel chains (hipGraph) per slot: a
+ * resident submit (warpjq_amd_submit_device / _spans) whose span table
+ * matches an earlier one under the same program, claim and arena replays it
+ * with one launch. 0 (the default) turns capture off. A serving knob for
+ * queries repeated over the same data; results do not depend on it. No
+ * counterpart in the reference. */
+warpjq_status warpjq_amd_set_graphs(warpjq_ctx *ctx, uint32_t per_slot);
+
 /* Workgroups (one wave each) the one-pass kernel launches: CUs times the
  * resident workgroups per CU for the context's arena. */
 uint32_t warpjq_amd_grid(const warpjq_ctx *ctx);

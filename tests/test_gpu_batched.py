@@ -203,3 +203,54 @@ def test_submit_spans_claim_sizes():
         finally:
             ctx.close()
     del keep
+
+
+def _norm(r, key):
+    d = w.result_dict(r)
+    gs = []
+    for g in d["groups"]:
+        g = dict(g)
+        g["key"] = key(g["key_off"], g["key_len"]) if g["key_kind"] > 1 else b""
+        del g["key_off"]  # where the key bytes sit, not what they are
+        gs.append(tuple(sorted(g.items())))
+    d["groups"] = sorted(gs)
+    return d
+
+
+@pytest.mark.parametrize("q", ["select(.status == 500) | count", "group_by(.host) | sum(.bytes)",
+                               "{h: .host, s: .status}", "sum(.bytes)"])
+def test_graph_replays_match_eager(q):
+    # captured chains replayed many times, with eager work and syncs between
+    # the replays, give the eager result every time
+    da, db = corpus.nginx(41, 3000), corpus.hostile(42, 800, 0.05)
+    A, B = w.DeviceBuffer(da), w.DeviceBuffer(db)
+    eager = w.Context(q, chunk_bytes=1 << 22)
+    g = w.Context(q, chunk_bytes=1 << 22)
+    g.set_graphs(2)
+
+    def run(ctx, slot, submit):
+        submit(ctx, slot)
+        if submit is sab:  # a batched submit: keys in the slot's key copy
+            return _norm(ctx.wait(slot), lambda o, n: ctx.group_key(slot, o, n))
+        data = da if submit is sa else db
+        return _norm(ctx.wait(slot), lambda o, n: data[o:o + n])
+
+    sa = lambda c, k: c.submit_device(k, A.ptr, len(da))
+    sb = lambda c, k: c.submit_device(k, B.ptr, len(db))
+    sab = lambda c, k: c.submit_spans(k, [(A.ptr, len(da)), (B.ptr, len(db))])
+    try:
+        want_a, want_b = run(eager, 0, sa), run(eager, 0, sb)
+        want_ab = run(eager, 1, sab)
+        for k in range(5):
+            assert run(g, k & 1, sa) == want_a, k
+            assert run(eager, 0, sb) == want_b
+            assert run(g, k & 1, sb) == want_b, k
+            assert run(g, k & 1, sab) == want_ab, k  # three tables, two cached: some captured again
+        g.set_claim(4096)  # a new generation: every table captured again
+        for k in range(3):
+            assert run(g, 0, sa) == want_a
+    finally:
+        g.close()
+        eager.close()
+        A.free()
+        B.free()

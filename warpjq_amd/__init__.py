@@ -106,7 +106,7 @@ ADDITIVE_SYMBOLS = ["warpjq_amd_ext_value", "warpjq_amd_wait_ext", "warpjq_amd_s
                     "warpjq_amd_run_files", "warpjq_amd_device_upload", "warpjq_amd_device_free",
                     "warpjq_amd_server_create", "warpjq_amd_server_destroy", "warpjq_amd_server_add_dataset",
                     "warpjq_amd_server_submit", "warpjq_amd_server_wait", "warpjq_amd_server_get_stats",
-                    "warpjq_amd_ctx_reprogram",
+                    "warpjq_amd_ctx_reprogram", "warpjq_amd_set_graphs",
                     "warpjq_amd_free", "warpjq_amd_compile", "warpjq_amd_program_view",
                     "warpjq_amd_program_free", "warpjq_amd_gen_nginx"]
 
@@ -165,6 +165,7 @@ def lib():
         L.warpjq_amd_run_files.restype = ctypes.c_int
         L.warpjq_amd_submit_pieces.argtypes = [ctypes.c_void_p, u32, u64, ctypes.c_void_p, u32, cp, sz]
         L.warpjq_amd_ctx_reprogram.argtypes = [ctypes.c_void_p, P(Program), cp, sz]
+        L.warpjq_amd_set_graphs.argtypes = [ctypes.c_void_p, u32]
         L.warpjq_amd_device_upload.argtypes = [ctypes.c_void_p, u64, cp, sz]
         L.warpjq_amd_device_upload.restype = ctypes.c_void_p
         L.warpjq_amd_device_free.argtypes = [ctypes.c_void_p]
@@ -335,6 +336,11 @@ class Context:
         if rc != 0:
             raise DeviceError(f"reprogram: {STATUS_NAMES.get(rc, rc)}: {e.value.decode()}")
         self.prog = prog
+
+    def set_graphs(self, per_slot):
+        """Cache up to `per_slot` captured chains (hipGraph) per slot for resident submits."""
+        if lib().warpjq_amd_set_graphs(self.ctx, per_slot) != 0:
+            raise ValueError("set_graphs: at most 65536 per slot")
 
     def set_claim(self, nbytes):
         if lib().warpjq_amd_set_claim(self.ctx, nbytes) != 0:

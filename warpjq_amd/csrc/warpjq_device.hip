@@ -361,6 +361,14 @@ __global__ void k_res_records(DevCtr *ctr, const unsigned long long *prefix, uin
 
 namespace {
 
+// A kernel chain captured for one span table and program generation.
+struct GraphEntry {
+  std::vector<Span> spans;
+  uint64_t gen;
+  hipGraphExec_t exec;
+  uint64_t last;
+};
+
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;  // before H2D, after H2D, after kernels
@@ -396,6 +404,8 @@ struct Slot {
   Span *d_spans = nullptr, *h_spans = nullptr;
   warpjq_amd_span *h_pieces = nullptr;
   uint32_t *d_long = nullptr;  // deferred long lines, 3 words each, zero between launches
+  std::vector<GraphEntry> graphs;  // captured chains of resident submits
+  uint64_t graph_tick = 0;
   bool keys_blob = false;  // group keys copied out (batched submits)
   uint32_t n_spans = 0;
   uint8_t *d_keyblob = nullptr, *h_keyblob = nullptr;
@@ -425,6 +435,8 @@ struct warpjq_ctx {
   uint32_t max_tiles = 0;
   uint32_t claim = CLAIM_DEFAULT;
   uint32_t arena = ARENA_DEFAULT;
+  uint32_t graph_cap = 0;  // chains cached per slot (0: no graphs)
+  uint64_t prog_gen = 0;   // bumped when anything a captured chain bakes in changes
   bool grouped = false, rows = false;
   int device = 0;
 };
@@ -493,6 +505,8 @@ static void free_slot(Slot &s) {
   (void)hipFree(s.d_fb_tile);
   (void)hipFree(s.d_spans);
   (void)hipFree(s.d_long);
+  for (GraphEntry &g : s.graphs) (void)hipGraphExecDestroy(g.exec);
+  s.graphs.clear();
   (void)hipHostFree(s.h_spans);
   delete[] s.h_pieces;
   (void)hipHostFree(s.h_fb_span);
@@ -770,6 +784,7 @@ warpjq_status warpjq_amd_ctx_reprogram(warpjq_ctx *ctx, const warpjq_program *pr
   ctx->host = host;
   warpjq_status st = put_program(ctx, prog, err, err_len);
   if (st != WARPJQ_OK) ctx->host = old;  // tables too large: the old program stays
+  ctx->prog_gen++;
   return st;
 }
 
@@ -784,33 +799,13 @@ uint64_t warpjq_max_lines(const warpjq_ctx *ctx) { return ctx ? ctx->max_lines :
 
 }  // extern "C"
 
-// Queues the kernel chain over the device spans `in[0..n_in)` on `s`.
-static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const warpjq_amd_span *in, uint32_t n_in, char *err,
-                                  size_t err_len) {
+// Queues the kernel chain over the span table already in s.h_spans[0..ns)
+// on the slot's stream. `timed`: hipEvents around the main kernel (left out
+// of a captured graph).
+static warpjq_status enqueue_chain(warpjq_ctx *ctx, Slot &s, uint32_t ns, uint32_t ntiles, uint64_t total, bool timed,
+                                   char *err, size_t err_len) {
   hipStream_t st = s.stream;
   const uint32_t claim = ctx->claim;
-  // place the spans in the kernels' position space, each on a claim boundary
-  uint64_t v = 0, total = 0;
-  uint32_t ns = 0;
-  for (uint32_t i = 0; i < n_in; i++) {
-    if (in[i].n_bytes == 0) continue;
-    const uint32_t mis = (uint32_t)((uintptr_t)in[i].d_bytes & 15u);
-    Span sp;
-    sp.base = in[i].d_bytes - mis;
-    sp.vbase = (uint32_t)v;
-    sp.start = (uint32_t)(v + mis);
-    sp.vend = (uint32_t)(v + mis + in[i].n_bytes);
-    sp._pad = 0;
-    s.h_spans[ns++] = sp;
-    total += in[i].n_bytes;
-    v = ((v + mis + in[i].n_bytes + claim - 1) / claim) * claim;
-  }
-  if (ns == 0) {  // keep one empty span so results can be read uniformly
-    s.h_spans[0] = Span{in && n_in ? in[0].d_bytes : nullptr, 0, 0, 0, 0};
-  }
-  s.n_spans = ns ? ns : 1;
-  const uint32_t ntiles = (uint32_t)(v / claim);
-  s.ntiles = ntiles;
   hipLaunchKernelGGL(k_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (ntiles + 255) / 256))), dim3(256),
                      0, st, s.d_ctr, s.d_tile_state, ntiles);
   if (total > 0) {
@@ -839,7 +834,7 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const warpjq_amd_spa
     a.long_q = s.d_long;
     a.long_cap = ctx->max_tiles;
     uint32_t grid = std::min<uint32_t>(ctx->grid, ntiles);
-    HIP_TRY(hipEventRecord(s.evm0, st), "hipEventRecord");
+    if (timed) HIP_TRY(hipEventRecord(s.evm0, st), "hipEventRecord");
     switch (ctx->host.ns) {
       case 1: launch_main_1(ctx->grouped, ctx->rows, grid, st, a); break;
       case 2: launch_main_2(ctx->grouped, ctx->rows, grid, st, a); break;
@@ -848,7 +843,7 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const warpjq_amd_spa
       default: launch_main_16(ctx->grouped, ctx->rows, grid, st, a); break;
     }
     HIP_TRY(hipGetLastError(), "k_main");
-    HIP_TRY(hipEventRecord(s.evm1, st), "hipEventRecord");
+    if (timed) HIP_TRY(hipEventRecord(s.evm1, st), "hipEventRecord");
     if (ctx->dprog.output_kind == WARPJQ_OUT_AGG) {
       if (ctx->grouped) {
         hipLaunchKernelGGL(k_group_compact, dim3(1024), dim3(256), 0, st, s.gt, s.d_ctr, s.d_groups, s.d_gext, s.d_spans,
@@ -881,6 +876,74 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const warpjq_amd_spa
   HIP_TRY(hipMemcpyAsync(s.h_ctr, s.d_ctr, sizeof(DevCtr), hipMemcpyDeviceToHost, st), "D2H(counters)");
   if (ctx->dprog.output_kind == WARPJQ_OUT_AGG && !ctx->grouped)
     HIP_TRY(hipMemcpyAsync(s.h_aggout, s.d_aggout, sizeof(AggOut), hipMemcpyDeviceToHost, st), "D2H(agg)");
+  return WARPJQ_OK;
+}
+
+// Queues the kernel chain over the device spans `in[0..n_in)` on `s`.
+static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const warpjq_amd_span *in, uint32_t n_in, char *err,
+                                  size_t err_len) {
+  hipStream_t st = s.stream;
+  const uint32_t claim = ctx->claim;
+  // place the spans in the kernels' position space, each on a claim boundary
+  uint64_t v = 0, total = 0;
+  uint32_t ns = 0;
+  for (uint32_t i = 0; i < n_in; i++) {
+    if (in[i].n_bytes == 0) continue;
+    const uint32_t mis = (uint32_t)((uintptr_t)in[i].d_bytes & 15u);
+    Span sp;
+    sp.base = in[i].d_bytes - mis;
+    sp.vbase = (uint32_t)v;
+    sp.start = (uint32_t)(v + mis);
+    sp.vend = (uint32_t)(v + mis + in[i].n_bytes);
+    sp._pad = 0;
+    s.h_spans[ns++] = sp;
+    total += in[i].n_bytes;
+    v = ((v + mis + in[i].n_bytes + claim - 1) / claim) * claim;
+  }
+  if (ns == 0) {  // keep one empty span so results can be read uniformly
+    s.h_spans[0] = Span{in && n_in ? in[0].d_bytes : nullptr, 0, 0, 0, 0};
+  }
+  s.n_spans = ns ? ns : 1;
+  const uint32_t ntiles = (uint32_t)(v / claim);
+  s.ntiles = ntiles;
+  // a resident submit seen before on this slot: replay its captured chain
+  if (ctx->graph_cap && s.resident && total > 0) {
+    GraphEntry *hit = nullptr;
+    for (GraphEntry &g : s.graphs)
+      if (g.gen == ctx->prog_gen && g.spans.size() == ns &&
+          memcmp(g.spans.data(), s.h_spans, (size_t)ns * sizeof(Span)) == 0)
+        hit = &g;
+    if (!hit) {
+      HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      warpjq_status cs = enqueue_chain(ctx, s, ns, ntiles, total, false, err, err_len);
+      hipGraph_t graph = nullptr;
+      hipError_t ce = hipStreamEndCapture(st, &graph);
+      if (cs != WARPJQ_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return cs;
+      }
+      HIP_TRY(ce, "hipStreamEndCapture");
+      hipGraphExec_t exec = nullptr;
+      hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIP_TRY(ie, "hipGraphInstantiate");
+      if (s.graphs.size() >= ctx->graph_cap) {  // drop the least recently used
+        auto old = std::min_element(s.graphs.begin(), s.graphs.end(),
+                                    [](const GraphEntry &a, const GraphEntry &b) { return a.last < b.last; });
+        (void)hipGraphExecDestroy(old->exec);
+        s.graphs.erase(old);
+      }
+      s.graphs.push_back(GraphEntry{std::vector<Span>(s.h_spans, s.h_spans + ns), ctx->prog_gen, exec, 0});
+      hit = &s.graphs.back();
+    }
+    hit->last = ++s.graph_tick;
+    HIP_TRY(hipEventRecord(s.evm0, st), "hipEventRecord");
+    HIP_TRY(hipGraphLaunch(hit->exec, st), "hipGraphLaunch");
+    HIP_TRY(hipEventRecord(s.evm1, st), "hipEventRecord");
+  } else {
+    warpjq_status es = enqueue_chain(ctx, s, ns, ntiles, total, true, err, err_len);
+    if (es != WARPJQ_OK) return es;
+  }
   HIP_TRY(hipEventRecord(s.ev2, st), "hipEventRecord");
   s.submitted = true;
   return WARPJQ_OK;
@@ -1139,6 +1202,7 @@ warpjq_status warpjq_amd_set_claim(warpjq_ctx *ctx, uint32_t claim_bytes) {
   if (!ctx || claim_bytes < CLAIM_MIN || claim_bytes > CLAIM_MAX || claim_bytes % STEP_BYTES) return WARPJQ_ERR_INVALID_ARG;
   if (claim_bytes + TAIL_MAX > ctx->arena) return WARPJQ_ERR_INVALID_ARG;
   ctx->claim = claim_bytes;
+  ctx->prog_gen++;
   return WARPJQ_OK;
 }
 
@@ -1147,9 +1211,23 @@ warpjq_status warpjq_amd_set_arena(warpjq_ctx *ctx, uint32_t arena_bytes) {
     return WARPJQ_ERR_INVALID_ARG;
   ctx->arena = arena_bytes;
   ctx->grid = main_grid(ctx);
+  ctx->prog_gen++;
   return WARPJQ_OK;
 }
 
 uint32_t warpjq_amd_grid(const warpjq_ctx *ctx) { return ctx ? ctx->grid : 0; }
+
+warpjq_status warpjq_amd_set_graphs(warpjq_ctx *ctx, uint32_t per_slot) {
+  if (!ctx || per_slot > 65536) return WARPJQ_ERR_INVALID_ARG;
+  ctx->graph_cap = per_slot;
+  for (uint32_t i = 0; i < ctx->n_slots; i++) {
+    Slot &s = ctx->slots[i];
+    while (s.graphs.size() > per_slot) {
+      (void)hipGraphExecDestroy(s.graphs.front().exec);
+      s.graphs.erase(s.graphs.begin());
+    }
+  }
+  return WARPJQ_OK;
+}
 
 }  // extern "C"
