@@ -339,6 +339,7 @@ def main():
     ap.add_argument("--cpu-sample-mib", type=int, default=1024)
     ap.add_argument("--claim-kib", type=int, default=0, help="bytes per wave claim (tuning)")
     ap.add_argument("--arena-kib", type=int, default=0, help="LDS arena per wave (tuning)")
+    ap.add_argument("--autotune", action="store_true", help="pick claim / arena with the autotuner first")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy the input once device-to-device (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -379,6 +380,11 @@ def main():
         ctx.set_arena(args.arena_kib << 10)
     if args.claim_kib:
         ctx.set_claim(args.claim_kib << 10)
+    tuned = None
+    if args.autotune:
+        c_, a_, ms_ = ctx.autotune(base, pieces[0][1] - pieces[0][0])
+        tuned = {"claim": c_, "arena": a_, "sample_main_ms": round(ms_, 3)}
+        log("autotune", tuned)
 
     def step():
         # one stream: pieces run back to back, so per-kernel events are clean
@@ -492,7 +498,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "nginx NDJSON, select(.status == 500) | count, resident in HBM",
                    "bytes": n, "lines": n_lines, "avg_line_bytes": round(n / n_lines, 1),
-                   "pieces": len(pieces), "device": dev, "grid": ctx.grid},
+                   "pieces": len(pieces), "device": dev, "grid": ctx.grid, "autotuned": tuned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "kernel": "k_main", "main_ms_per_step": round(main_ms, 3)},

@@ -119,6 +119,15 @@ warpjq_status warpjq_amd_set_claim(warpjq_ctx *ctx, uint32_t claim_bytes);
  * No counterpart in the reference. */
 warpjq_status warpjq_amd_set_arena(warpjq_ctx *ctx, uint32_t arena_bytes);
 
+/* Autotuner: times the one-pass kernel on the first 1 GiB of the resident
+ * bytes (the slot capacity for row outputs) under a dozen claim / arena
+ * pairs, keeps the fastest for this context and returns it with its time.
+ * Uses slot 0, which must be idle. Results do not depend on the choice. No
+ * counterpart in the reference (its tunables are compile-time constants,
+ * kernels.cu:52-89). */
+warpjq_status warpjq_amd_autotune(warpjq_ctx *ctx, const uint8_t *d_bytes, uint64_t n_bytes, uint32_t *claim_out,
+                                  uint32_t *arena_out, float *best_ms_out, char *err, size_t err_len);
+
 /* Keep up to `per_slot` captured kernel chains (hipGraph) per slot: a
  * resident submit (warpjq_amd_submit_device / _spans) whose span table
  * matches an earlier one under the same program, claim and arena replays it

@@ -254,3 +254,28 @@ def test_graph_replays_match_eager(q):
         eager.close()
         A.free()
         B.free()
+
+
+def test_autotune_keeps_results(tmp_path):
+    data = corpus.nginx(51, 5000)
+    buf = w.DeviceBuffer(data)
+    q = "select(.status == 500) | count"
+    ctx = w.Context(q, chunk_bytes=1 << 22)
+    try:
+        ctx.submit_device(0, buf.ptr, len(data))
+        want = w.result_dict(ctx.wait(0))
+        cache = w.TuneCache(str(tmp_path / "tune.json"))
+        claim, arena, ms = ctx.autotune(buf.ptr, len(data), cache)
+        assert ms is not None and ms > 0
+        assert 1024 <= claim <= 8192 and arena >= claim + 1024
+        ctx.submit_device(0, buf.ptr, len(data))
+        assert w.result_dict(ctx.wait(0)) == want
+        # a second context takes the stored choice without sweeping
+        c2 = w.Context(q, chunk_bytes=1 << 22)
+        assert c2.autotune(buf.ptr, len(data), w.TuneCache(str(tmp_path / "tune.json"))) == (claim, arena, None)
+        c2.submit_device(0, buf.ptr, len(data))
+        assert w.result_dict(c2.wait(0)) == want
+        c2.close()
+    finally:
+        ctx.close()
+        buf.free()

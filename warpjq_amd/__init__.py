@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libwarpjq_amd.so")
 
 FMT_NDJSON, FMT_CSV, FMT_COUNT = 0, 1, 2
 WARN, SKIP, ABORT = 0, 1, 2
+OUT_AGG = 3  # warpjq_out_kind (warpjq_kernels.h)
 GPU_CHUNK = 64 << 20          # gpu/mod.rs:57
 MAX_LINE = 64 << 20           # chunk.rs:22
 NONE = 0xFFFFFFFF
@@ -106,7 +107,7 @@ ADDITIVE_SYMBOLS = ["warpjq_amd_ext_value", "warpjq_amd_wait_ext", "warpjq_amd_s
                     "warpjq_amd_run_files", "warpjq_amd_device_upload", "warpjq_amd_device_free",
                     "warpjq_amd_server_create", "warpjq_amd_server_destroy", "warpjq_amd_server_add_dataset",
                     "warpjq_amd_server_submit", "warpjq_amd_server_wait", "warpjq_amd_server_get_stats",
-                    "warpjq_amd_ctx_reprogram", "warpjq_amd_set_graphs",
+                    "warpjq_amd_ctx_reprogram", "warpjq_amd_set_graphs", "warpjq_amd_autotune",
                     "warpjq_amd_free", "warpjq_amd_compile", "warpjq_amd_program_view",
                     "warpjq_amd_program_free", "warpjq_amd_gen_nginx"]
 
@@ -166,6 +167,8 @@ def lib():
         L.warpjq_amd_submit_pieces.argtypes = [ctypes.c_void_p, u32, u64, ctypes.c_void_p, u32, cp, sz]
         L.warpjq_amd_ctx_reprogram.argtypes = [ctypes.c_void_p, P(Program), cp, sz]
         L.warpjq_amd_set_graphs.argtypes = [ctypes.c_void_p, u32]
+        L.warpjq_amd_autotune.argtypes = [ctypes.c_void_p, ctypes.c_void_p, u64, P(u32), P(u32),
+                                          P(ctypes.c_float), cp, sz]
         L.warpjq_amd_device_upload.argtypes = [ctypes.c_void_p, u64, cp, sz]
         L.warpjq_amd_device_upload.restype = ctypes.c_void_p
         L.warpjq_amd_device_free.argtypes = [ctypes.c_void_p]
@@ -319,6 +322,7 @@ class Context:
         if rc != 0:
             raise DeviceError(f"ctx_create: {STATUS_NAMES.get(rc, rc)}: {e.value.decode()}")
         self.n_slots = n_slots
+        self._claim, self._arena = 8 << 10, 18 << 10  # the library's defaults
 
     def close(self):
         if self.ctx:
@@ -337,6 +341,35 @@ class Context:
             raise DeviceError(f"reprogram: {STATUS_NAMES.get(rc, rc)}: {e.value.decode()}")
         self.prog = prog
 
+    def autotune(self, dptr, n, cache=None):
+        """Times claim / arena pairs on resident bytes and keeps the fastest.
+        With `cache` (a TuneCache), a stored choice for the same device,
+        kernel shape and line-length class is applied instead of sweeping.
+        Returns (claim, arena, main_ms or None when taken from the cache)."""
+        key = None
+        if cache is not None:
+            key = cache.key(self, dptr, n)
+            hit = cache.get(key)
+            if hit:
+                # the arena first when it grows, so the claim always fits
+                if hit[1] >= self._arena:
+                    self.set_arena(hit[1])
+                    self.set_claim(hit[0])
+                else:
+                    self.set_claim(hit[0])
+                    self.set_arena(hit[1])
+                self._claim, self._arena = hit
+                return hit[0], hit[1], None
+        c, a, ms = u32(), u32(), ctypes.c_float()
+        e = _err()
+        rc = lib().warpjq_amd_autotune(self.ctx, dptr, n, ctypes.byref(c), ctypes.byref(a), ctypes.byref(ms), e, 512)
+        if rc != 0:
+            raise DeviceError(f"autotune: {STATUS_NAMES.get(rc, rc)}: {e.value.decode()}")
+        self._claim, self._arena = c.value, a.value
+        if cache is not None:
+            cache.put(key, (c.value, a.value))
+        return c.value, a.value, ms.value
+
     def set_graphs(self, per_slot):
         """Cache up to `per_slot` captured chains (hipGraph) per slot for resident submits."""
         if lib().warpjq_amd_set_graphs(self.ctx, per_slot) != 0:
@@ -345,10 +378,12 @@ class Context:
     def set_claim(self, nbytes):
         if lib().warpjq_amd_set_claim(self.ctx, nbytes) != 0:
             raise ValueError("claim must be 1-8 KiB in 1 KiB steps and fit the arena with 1 KiB to spare")
+        self._claim = nbytes
 
     def set_arena(self, nbytes):
         if lib().warpjq_amd_set_arena(self.ctx, nbytes) != 0:
             raise ValueError("arena must be a multiple of 256 B, at most 32 KiB, at least the claim plus 1 KiB")
+        self._arena = nbytes
 
     @property
     def grid(self):
@@ -417,6 +452,45 @@ class Context:
         a, b, m = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
         lib().warpjq_amd_slot_timing(self.ctx, slot, ctypes.byref(a), ctypes.byref(b), ctypes.byref(m))
         return a.value, b.value, m.value
+
+
+class TuneCache:
+    """Autotuner choices persisted as JSON: (device, kernel width, output
+    shape, line-length class) -> (claim, arena)."""
+
+    def __init__(self, path):
+        import json
+        self.path = path
+        try:
+            with open(path) as f:
+                self.d = json.load(f)
+        except (OSError, ValueError):
+            self.d = {}
+
+    def key(self, ctx, dptr, n):
+        # line-length class from one pass over (a prefix of) the bytes
+        m = min(n, 16 << 20)
+        ctx.submit_device(0, dptr, m)
+        r = ctx.wait(0)
+        avg = m / max(1, r.n_lines)
+        cls = 0
+        while (64 << cls) < avg and cls < 16:
+            cls += 1
+        pv = ctx.prog.view.contents
+        shape = "rows" if pv.output_kind != OUT_AGG else ("groups" if pv.group_path != 0xFFFFFFFF else "agg")
+        return f"{device_name()}|w{lib().warpjq_amd_kernel_width(ctx.ctx)}|{shape}|{64 << cls}"
+
+    def get(self, key):
+        v = self.d.get(key)
+        return tuple(v) if v else None
+
+    def put(self, key, v):
+        import json
+        self.d[key] = list(v)
+        tmp = self.path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(self.d, f, indent=1, sort_keys=True)
+        os.replace(tmp, self.path)
 
 
 class DeviceBuffer:

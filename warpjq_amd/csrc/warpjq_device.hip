@@ -1217,6 +1217,57 @@ warpjq_status warpjq_amd_set_arena(warpjq_ctx *ctx, uint32_t arena_bytes) {
 
 uint32_t warpjq_amd_grid(const warpjq_ctx *ctx) { return ctx ? ctx->grid : 0; }
 
+warpjq_status warpjq_amd_autotune(warpjq_ctx *ctx, const uint8_t *d_bytes, uint64_t n_bytes, uint32_t *claim_out,
+                                  uint32_t *arena_out, float *best_ms_out, char *err, size_t err_len) {
+  if (!ctx || (!d_bytes && n_bytes)) return WARPJQ_ERR_INVALID_ARG;
+  // claim and arena pairs around the default; the arena sets the workgroups
+  // per CU, the claim the bytes per ticket and the fill granularity
+  static const uint32_t cand[][2] = {{8, 18}, {8, 16}, {8, 20}, {8, 24}, {7, 16}, {6, 14}, {6, 16},
+                                     {4, 12}, {4, 8},  {8, 32}, {5, 12}, {8, 12}};
+  const uint64_t sample = std::min<uint64_t>(n_bytes, ctx->rows ? ctx->chunk_cap : (1ull << 30));
+  const uint32_t claim0 = ctx->claim, arena0 = ctx->arena;
+  float best = 1e30f;
+  uint32_t bc = claim0, ba = arena0;
+  for (const auto &c : cand) {
+    const uint32_t cl = c[0] << 10, ar = c[1] << 10;
+    if (cl + TAIL_MAX > ar || ar > ARENA_MAX) continue;
+    ctx->arena = ar;  // both set together: the order of the two setters would matter
+    ctx->claim = cl;
+    ctx->grid = main_grid(ctx);
+    ctx->prog_gen++;
+    float t = 1e30f;
+    for (int rep = 0; rep < 3; rep++) {
+      warpjq_status st = warpjq_amd_submit_device(ctx, 0, d_bytes, sample, err, err_len);
+      if (st == WARPJQ_OK) {
+        warpjq_chunk_result r;
+        st = warpjq_wait(ctx, 0, &r, err, err_len);
+      }
+      if (st != WARPJQ_OK) {
+        ctx->claim = claim0;
+        ctx->arena = arena0;
+        ctx->grid = main_grid(ctx);
+        return st;
+      }
+      float m = 0.f;
+      warpjq_amd_slot_timing(ctx, 0, nullptr, nullptr, &m);
+      if (rep && m < t) t = m;  // the first run warms caches and code
+    }
+    if (t < best) {
+      best = t;
+      bc = cl;
+      ba = ar;
+    }
+  }
+  ctx->claim = bc;
+  ctx->arena = ba;
+  ctx->grid = main_grid(ctx);
+  ctx->prog_gen++;
+  if (claim_out) *claim_out = bc;
+  if (arena_out) *arena_out = ba;
+  if (best_ms_out) *best_ms_out = best;
+  return WARPJQ_OK;
+}
+
 warpjq_status warpjq_amd_set_graphs(warpjq_ctx *ctx, uint32_t per_slot) {
   if (!ctx || per_slot > 65536) return WARPJQ_ERR_INVALID_ARG;
   ctx->graph_cap = per_slot;
