@@ -222,6 +222,13 @@ void warpjq_amd_program_free(warpjq_amd_program *p);
 int warpjq_amd_gen_nginx(uint64_t seed, uint64_t target_bytes, uint32_t threads, uint64_t pad, uint8_t **out,
                          uint64_t *n_bytes, uint64_t *n_lines, uint64_t *status_hist);
 
+/* The same for the nested schema (~330 B/line, objects six deep, an escaped
+ * key, .user drawn from 1M values with Zipf s = 1.1, .bytes an integer on
+ * 70% of lines / a %.17g double on 28% / null on 2%, .resp.latency; 0.1% of
+ * lines malformed, counted in *n_malformed). No counterpart in the reference. */
+int warpjq_amd_gen_nested(uint64_t seed, uint64_t target_bytes, uint32_t threads, uint64_t pad, uint8_t **out,
+                          uint64_t *n_bytes, uint64_t *n_lines, uint64_t *n_malformed);
+
 #ifdef __cplusplus
 }
 #endif

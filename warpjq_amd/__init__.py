@@ -109,7 +109,7 @@ ADDITIVE_SYMBOLS = ["warpjq_amd_ext_value", "warpjq_amd_wait_ext", "warpjq_amd_s
                     "warpjq_amd_server_submit", "warpjq_amd_server_wait", "warpjq_amd_server_get_stats",
                     "warpjq_amd_ctx_reprogram", "warpjq_amd_set_graphs", "warpjq_amd_autotune",
                     "warpjq_amd_free", "warpjq_amd_compile", "warpjq_amd_program_view",
-                    "warpjq_amd_program_free", "warpjq_amd_gen_nginx"]
+                    "warpjq_amd_program_free", "warpjq_amd_gen_nginx", "warpjq_amd_gen_nested"]
 
 
 def build(jobs=8):
@@ -196,6 +196,8 @@ def lib():
         L.warpjq_amd_program_free.restype = None
         L.warpjq_amd_gen_nginx.argtypes = [u64, u64, u32, u64, P(P(ctypes.c_uint8)), P(u64), P(u64), P(u64)]
         L.warpjq_amd_gen_nginx.restype = ctypes.c_int
+        L.warpjq_amd_gen_nested.argtypes = [u64, u64, u32, u64, P(P(ctypes.c_uint8)), P(u64), P(u64), P(u64)]
+        L.warpjq_amd_gen_nested.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -588,6 +590,24 @@ def gen_nginx(target_bytes, seed=1, threads=None, pad=256):
     ctypes.memmove(arr.ctypes.data, out, total)
     lib().warpjq_amd_free(out)
     return arr, n.value, lines.value, dict(zip(STATUS_ORDER, list(hist)))
+
+
+def gen_nested(target_bytes, seed=1, threads=None, pad=256):
+    """Synthetic nested-schema NDJSON. Returns (numpy uint8 array of n+pad
+    bytes, n_bytes, n_lines, n_malformed)."""
+    import numpy as np
+    threads = threads or min(16, os.cpu_count() or 1)
+    out = P(ctypes.c_uint8)()
+    n, lines, bad = u64(), u64(), u64()
+    rc = lib().warpjq_amd_gen_nested(seed, target_bytes, threads, pad, ctypes.byref(out), ctypes.byref(n),
+                                     ctypes.byref(lines), ctypes.byref(bad))
+    if rc != 0:
+        raise MemoryError("generator allocation failed")
+    total = n.value + pad
+    arr = np.empty(total, dtype=np.uint8)
+    ctypes.memmove(arr.ctypes.data, out, total)
+    lib().warpjq_amd_free(out)
+    return arr, n.value, lines.value, bad.value
 
 
 def result_dict(r):

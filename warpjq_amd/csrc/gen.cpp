@@ -113,9 +113,170 @@ int nginx_line(uint64_t seed, uint64_t i, std::string &out) {
   return si;
 }
 
+// nested schema (~330 bytes, objects six deep):
+//   {"ts":T,"user":U,"bytes":B,"resp":{"latency":L,"code":C,"meta":{"region":R,
+//    "tags":[..],"d":{"e":{"g":{"f":K}}}}},"x\u002did":"..","ok":true|false}
+//   user    "u" + 11 base-36 chars, 1M distinct values, Zipf s = 1.1; the key
+//           is written "us\u0065r" on 2% of lines
+//   bytes   an integer floor(lognormal(10, 3)) capped at 2^53 on 70% of
+//           lines, %.17g of U[0, 1e6) on 28%, null on 2%
+//   latency %.3f of lognormal(3, 1)
+//   0.1% of lines are malformed: a leading zero, NaN, or the line cut short
+constexpr uint32_t N_USERS = 1u << 20;
+
+const std::vector<double> &user_cdf() {
+  static const std::vector<double> cdf = [] {
+    std::vector<double> c(N_USERS);
+    double acc = 0;
+    for (uint32_t k = 0; k < N_USERS; k++) {
+      acc += std::pow((double)(k + 1), -1.1);
+      c[k] = acc;
+    }
+    for (auto &x : c) x /= acc;
+    c.back() = 1.0;
+    return c;
+  }();
+  return cdf;
+}
+
+const char *REGIONS[6] = {"us-east-1", "us-west-2", "eu-west-2", "ap-south-1", "sa-east-1", "eu-central-1"};
+const int CODES[6] = {200, 201, 204, 400, 404, 500};
+
+// Appends line `i`; returns 1 if it is malformed.
+int nested_line(uint64_t seed, uint64_t i, std::string &out) {
+  Rng r{seed ^ (i * 0xA24BAED4963EE407ull) ^ 0x5DEECE66Dull};
+  r.next();
+  const std::vector<double> &cdf = user_cdf();
+  const double u = r.uniform();
+  uint32_t lo = 0, hi = N_USERS - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (cdf[mid] < u) lo = mid + 1;
+    else hi = mid;
+  }
+  uint64_t x = ((uint64_t)lo * 0x9E3779B97F4A7C15ull) % 131621703842267136ull;  // 36^11
+  char user[13];
+  user[0] = 'u';
+  for (int k = 11; k >= 1; k--) {
+    const int dgt = (int)(x % 36);
+    user[k] = (char)(dgt < 10 ? '0' + dgt : 'a' + dgt - 10);
+    x /= 36;
+  }
+  user[12] = 0;
+  const bool esc_user = r.uniform() < 0.02;
+  char bytes[40];
+  const double bu = r.uniform();
+  if (bu < 0.70) {
+    double u1 = r.uniform(), u2 = r.uniform();
+    if (u1 < 1e-300) u1 = 1e-300;
+    const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    double b = std::floor(std::exp(10.0 + 3.0 * z));
+    if (b > 9007199254740992.0) b = 9007199254740992.0;
+    snprintf(bytes, sizeof bytes, "%llu", (unsigned long long)b);
+  } else if (bu < 0.98) {
+    snprintf(bytes, sizeof bytes, "%.17g", r.uniform() * 1e6);
+  } else {
+    snprintf(bytes, sizeof bytes, "null");
+  }
+  double u1 = r.uniform(), u2 = r.uniform();
+  if (u1 < 1e-300) u1 = 1e-300;
+  const double lat = std::exp(3.0 + std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2));
+  const int code = CODES[r.next() % 6];
+  const char *region = REGIONS[r.next() % 6];
+  const int ntags = (int)(r.next() % 4);
+  std::string tags;
+  for (int k = 0; k < ntags; k++) {
+    if (k) tags.push_back(',');
+    tags += "\"t" + std::to_string(r.next() % 50) + "\"";
+  }
+  const uint64_t xid = r.next();
+  const bool ok = (r.next() & 1) != 0;
+  const double bad = r.uniform();
+  int malformed = 0;
+  if (bad < 0.0003) {  // leading zero
+    snprintf(bytes, sizeof bytes, "0%llu", (unsigned long long)(1 + r.next() % 99999));
+    malformed = 1;
+  } else if (bad < 0.0006) {
+    snprintf(bytes, sizeof bytes, "NaN");
+    malformed = 1;
+  }
+  char buf[768];
+  int n = snprintf(buf, sizeof buf,
+                   "{\"ts\":%llu,\"%s\":\"%s\",\"bytes\":%s,\"resp\":{\"latency\":%.3f,\"code\":%d,\"meta\":{"
+                   "\"region\":\"%s\",\"tags\":[%s],\"d\":{\"e\":{\"g\":{\"f\":%d}}}}},"
+                   "\"x\\u002did\":\"%016llx\",\"ok\":%s}",
+                   (unsigned long long)(1700000000ull + i), esc_user ? "us\\u0065r" : "user", user, bytes, lat, code,
+                   region, tags.c_str(), (int)(r.next() % 1000), (unsigned long long)xid, ok ? "true" : "false");
+  if (bad >= 0.0006 && bad < 0.001) {  // cut short
+    n = 10 + (int)(r.next() % (uint64_t)(n - 20));
+    malformed = 1;
+  }
+  out.append(buf, n);
+  out.push_back('\n');
+  return malformed;
+}
+
 }  // namespace
 
 extern "C" {
+
+// Nested-schema lines, whole, until adding one more would pass target_bytes
+// (same contract as warpjq_amd_gen_nginx); *n_malformed counts the lines
+// made malformed on purpose.
+int warpjq_amd_gen_nested(uint64_t seed, uint64_t target_bytes, uint32_t threads, uint64_t pad, uint8_t **out,
+                          uint64_t *n_bytes, uint64_t *n_lines, uint64_t *n_malformed) {
+  if (threads == 0) threads = 1;
+  (void)user_cdf();
+  const uint64_t est_lines = target_bytes / 150 + 64;
+  const uint64_t per = (est_lines + threads - 1) / threads;
+  std::vector<std::string> parts(threads);
+  std::vector<std::vector<uint64_t>> ends(threads);
+  std::vector<std::vector<uint8_t>> bad(threads);
+  std::vector<std::thread> ts;
+  for (uint32_t t = 0; t < threads; t++) {
+    ts.emplace_back([&, t] {
+      uint64_t a = t * per, b = std::min<uint64_t>(est_lines, a + per);
+      parts[t].reserve((b - a) * 340);
+      for (uint64_t i = a; i < b; i++) {
+        bad[t].push_back((uint8_t)nested_line(seed, i, parts[t]));
+        ends[t].push_back(parts[t].size());
+        if (parts[t].size() > target_bytes) break;
+      }
+    });
+  }
+  for (auto &th : ts) th.join();
+  uint64_t total = 0, lines = 0, nbad = 0;
+  std::vector<uint64_t> take(threads, 0);
+  for (uint32_t t = 0; t < threads && total < target_bytes; t++) {
+    const auto &e = ends[t];
+    uint64_t lo = 0, hi = e.size();
+    while (lo < hi) {
+      uint64_t mid = (lo + hi) / 2;
+      if (total + e[mid] <= target_bytes) lo = mid + 1;
+      else hi = mid;
+    }
+    take[t] = lo ? e[lo - 1] : 0;
+    total += take[t];
+    lines += lo;
+    for (uint64_t k = 0; k < lo; k++) nbad += bad[t][k];
+    if (lo < e.size()) break;
+  }
+  uint8_t *buf = (uint8_t *)malloc(total + pad);
+  if (!buf) return 3;
+  uint64_t off = 0;
+  for (uint32_t t = 0; t < threads; t++) {
+    if (!take[t]) continue;
+    memcpy(buf + off, parts[t].data(), take[t]);
+    off += take[t];
+  }
+  memset(buf + total, 0, pad);
+  *out = buf;
+  *n_bytes = total;
+  *n_lines = lines;
+  *n_malformed = nbad;
+  return 0;
+}
+
 
 // Generates whole nginx-schema lines until adding one more would pass
 // target_bytes. *out is malloc'd (free with warpjq_amd_free) and padded with
