@@ -91,12 +91,18 @@ using namespace wjqd;
 // device-side structures
 // ---------------------------------------------------------------------------
 
+// Aggregate queries take claims from NREG regional tickets, GRAB claims per
+// atomic (device-scope atomics cost microseconds under contention).
+#define NREG 32
+#define RSTRIDE 32
+#define GRAB 4
+
 struct DevCtr {
   unsigned long long n_lines, n_blank, n_invalid, n_type_error, n_fallback, n_sel, out_total;
   unsigned long long first_bad;  // (claim << 32 | rank) of the first invalid line, aggregate queries
   unsigned int first_invalid, overflow, saw_non_numeric, group_overflow;
   unsigned int fb_count, n_groups, ticket, tiles_done;
-  unsigned int rticket[8];  // aggregate queries: one ticket per region of the chunk
+  unsigned int rticket[NREG * RSTRIDE];  // aggregate queries: one ticket per region, 128 B apart
   unsigned int blob_used;   // bytes of group keys copied out (multi-span submits)
   unsigned int long_count, long_next;  // deferred long lines: added, taken
 };
@@ -979,11 +985,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 
   // Claims: row outputs take them in chunk order from one ticket (the
   // look-back needs every earlier claim in flight); aggregate queries split
-  // the chunk into 8 regions with a ticket each, a wave starting in the
-  // region of its XCD (workgroups are dealt to the 8 XCDs round-robin) and
-  // moving on when it runs dry, which spreads the atomics and keeps a
-  // region's neighbouring claims in one XCD's L2.
-  uint32_t region = blockIdx.x & 7u, regions_tried = 0;
+  // the chunk into NREG regions with a ticket each and take GRAB consecutive
+  // claims per atomic. A wave starts in region blockIdx % NREG - a region of
+  // its own XCD, as workgroups are dealt to the 8 XCDs round-robin, so a
+  // region's neighbouring claims share one L2 - and moves on when it runs dry.
+  uint32_t region = blockIdx.x % NREG, regions_tried = 0;
+  // claims left from the last grab, [grab[0], grab[1]), kept in the claim
+  // ring's spare LDS words (SGPRs are all in use)
+  LDS_AS uint32_t *grab = Q.cr_base + CRING;
+  if (lane == 0) grab[0] = grab[1] = 0;
+  __syncthreads();
   auto take_claim = [&]() -> uint32_t {
     if (LB) {
       uint32_t tk = 0;
@@ -991,14 +1002,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       const uint32_t t = __shfl(tk, 0, 64);
       return t < a.ntiles ? t : ~0u;
     }
-    while (regions_tried < 8) {
-      const uint32_t lo = (uint32_t)(((uint64_t)region * a.ntiles) >> 3);
-      const uint32_t hi = (uint32_t)(((uint64_t)(region + 1) * a.ntiles) >> 3);
+    const uint32_t cur = grab[0], end = grab[1];
+    if (cur < end) {
+      if (lane == 0) grab[0] = cur + 1;
+      return cur;
+    }
+    while (regions_tried < NREG) {
+      const uint32_t lo = (uint32_t)(((uint64_t)region * a.ntiles) / NREG);
+      const uint32_t hi = (uint32_t)(((uint64_t)(region + 1) * a.ntiles) / NREG);
       uint32_t tk = 0;
-      if (lo < hi && lane == 0) tk = atomicAdd(&a.ctr->rticket[region], 1u);
+      if (lo < hi && lane == 0) tk = atomicAdd(&a.ctr->rticket[region * RSTRIDE], (unsigned)GRAB);
       tk = __shfl(tk, 0, 64);
-      if (lo + tk < hi) return lo + tk;
-      region = (region + 1) & 7u;
+      if (lo + tk < hi) {
+        if (lane == 0) {
+          grab[0] = lo + tk + 1;
+          grab[1] = min(lo + tk + GRAB, hi);
+        }
+        return lo + tk;
+      }
+      region = (region + 1) % NREG;
       regions_tried++;
     }
     return ~0u;

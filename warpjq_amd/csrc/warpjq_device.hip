@@ -17,10 +17,11 @@ __global__ void k_reset(DevCtr *ctr, unsigned long long *tile_state, uint32_t nt
     ctr->first_bad = ~0ull;
     ctr->overflow = ctr->saw_non_numeric = ctr->group_overflow = 0;
     ctr->fb_count = ctr->n_groups = ctr->ticket = ctr->tiles_done = 0;
-    for (int r = 0; r < 8; r++) ctr->rticket[r] = 0;
+
     ctr->blob_used = 0;
     ctr->long_count = ctr->long_next = 0;
   }
+  if (i < NREG) ctr->rticket[i * RSTRIDE] = 0;
   for (uint32_t t = i; t < ntiles; t += gridDim.x * blockDim.x) tile_state[t] = 0;
 }
 
