@@ -3,9 +3,12 @@
 //
 // Per submitted chunk, on the slot's stream:
 //   H2D (pinned staging -> HBM)
-//   k_reset        counters, tile states, ticket
+//   k_reset        counters, tile states, tickets
+//   k_line_count + k_tile_scan_*  (row outputs) line starts per claim and
+//                  their exclusive prefix: each claim's first line index
 //   k_main<NS>     ONE pass over the chunk. Persistent single-wave workgroups
-//                  claim tiles ("claims", a few KiB) in order from a ticket.
+//                  claim tiles ("claims", a few KiB) from regional tickets,
+//                  a few consecutive claims per atomic.
 //                  A claim is copied HBM -> LDS with direct-to-LDS 16-byte
 //                  loads (global_load_lds_dwordx4) into a per-wave LDS
 //                  "arena", followed by the first bytes of the next claim up
@@ -18,8 +21,8 @@
 //                  compacted before each new claim. Aggregates stay in
 //                  registers; group_by goes through a per-wave LDS table into
 //                  an HBM hash table. Row outputs need each line's index in
-//                  the chunk: the claim takes its first index by decoupled
-//                  look-back. Aggregate queries skip the look-back and store
+//                  the chunk: the claim reads its first index from the
+//                  pre-pass. Aggregate queries skip the pre-pass and store
 //                  per-claim line counts; the rare records that need an index
 //                  (first invalid line, declined lines) carry (claim, rank)
 //                  and are resolved after the pass.
@@ -91,8 +94,8 @@ using namespace wjqd;
 // device-side structures
 // ---------------------------------------------------------------------------
 
-// Aggregate queries take claims from NREG regional tickets, GRAB claims per
-// atomic (device-scope atomics cost microseconds under contention).
+// k_main takes claims from NREG regional tickets, GRAB claims per atomic
+// (device-scope atomics cost microseconds under contention).
 #define NREG 32
 #define RSTRIDE 32
 #define GRAB 4
@@ -102,7 +105,7 @@ struct DevCtr {
   unsigned long long first_bad;  // (claim << 32 | rank) of the first invalid line, aggregate queries
   unsigned int first_invalid, overflow, saw_non_numeric, group_overflow;
   unsigned int fb_count, n_groups, ticket, tiles_done;
-  unsigned int rticket[NREG * RSTRIDE];  // aggregate queries: one ticket per region, 128 B apart
+  unsigned int rticket[NREG * RSTRIDE];  // one claim ticket per region, 128 B apart
   unsigned int blob_used;   // bytes of group keys copied out (multi-span submits)
   unsigned int long_count, long_next;  // deferred long lines: added, taken
 };
@@ -142,7 +145,8 @@ struct KArgs {
   const uint8_t *tab;  // the program tables P points into (copied to LDS per workgroup)
   uint32_t tab_bytes;
   DevCtr *ctr;
-  // row outputs: look-back states; aggregates: line count per claim
+  // row outputs: each claim's first line index (line-count pre-pass);
+  // aggregates: line count per claim
   unsigned long long *tile_state;
   AggPart *parts;
   GroupTab gt;
@@ -414,13 +418,6 @@ __device__ __forceinline__ double from_ordered(uint64_t o) {
   return bits_to_f64(b);
 }
 
-__device__ __forceinline__ unsigned long long ld_acq(unsigned long long *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_rel(unsigned long long *p, unsigned long long v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Bit 7 of each byte set when that byte is '\n' (exact per byte).
 __device__ __forceinline__ uint32_t nl_bytes(uint32_t w) {
   const uint32_t x = w ^ 0x0A0A0A0Au;
@@ -442,41 +439,6 @@ __device__ __forceinline__ uint4 end_patch(uint4 v, uint32_t p, uint32_t n) {
       if (p + 4 * q + k >= n) w[q] = (w[q] & ~(0xFFu << (8 * k))) | (WJQ_END << (8 * k));
   }
   return make_uint4(w[0], w[1], w[2], w[3]);
-}
-
-#define FLAG_A (1ull << 62)
-#define FLAG_P (2ull << 62)
-#define VAL_MASK ((1ull << 62) - 1)
-
-// Decoupled look-back: publishes this tile's count and returns the sum of
-// counts of all earlier tiles. Wave-uniform.
-__device__ __forceinline__ uint64_t tile_lookback(unsigned long long *state, uint32_t t, uint64_t agg) {
-  uint32_t lane = lane_id();
-  if (t == 0) {
-    if (lane == 0) st_rel(&state[0], FLAG_P | agg);
-    return 0;
-  }
-  if (lane == 0) st_rel(&state[t], FLAG_A | agg);
-  uint64_t base = 0;
-  int64_t j = (int64_t)t - 1;
-  for (;;) {
-    int64_t idx = j - (int64_t)lane;
-    unsigned long long s = idx >= 0 ? ld_acq(&state[idx]) : FLAG_P;
-    uint64_t flag = s >> 62;
-    uint64_t pm = __ballot(flag == 2);
-    uint64_t zm = __ballot(flag == 0);
-    uint64_t rel = pm ? ((pm & (~pm + 1)) << 1) - 1 : ~0ull;  // lanes up to the first P
-    if (zm & rel) {
-      __builtin_amdgcn_s_sleep(2);
-      continue;
-    }
-    uint64_t v = ((rel >> lane) & 1) ? (s & VAL_MASK) : 0;
-    base += wave_sum(v);
-    if (pm) break;
-    j -= 64;
-  }
-  if (lane == 0) st_rel(&state[t], FLAG_P | (base + agg));
-  return base;
 }
 
 // ---------------------------------------------------------------------------
@@ -983,12 +945,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   uint32_t cseq = 0;  // claims given a ring slot so far
   uint32_t hseq = 0;  // the claim of the queue's head (when the queue is not empty)
 
-  // Claims: row outputs take them in chunk order from one ticket (the
-  // look-back needs every earlier claim in flight); aggregate queries split
-  // the chunk into NREG regions with a ticket each and take GRAB consecutive
-  // claims per atomic. A wave starts in region blockIdx % NREG - a region of
-  // its own XCD, as workgroups are dealt to the 8 XCDs round-robin, so a
-  // region's neighbouring claims share one L2 - and moves on when it runs dry.
+  // Claims: the chunk is split into NREG regions with a ticket each, and a
+  // wave takes GRAB consecutive claims per atomic. A wave starts in region
+  // blockIdx % NREG - a region of its own XCD, as workgroups are dealt to the
+  // 8 XCDs round-robin, so a region's neighbouring claims share one L2 - and
+  // moves on when it runs dry. Row outputs know each claim's first line index
+  // from the line-count pre-pass, so they take claims in any order too.
   uint32_t region = blockIdx.x % NREG, regions_tried = 0;
   // claims left from the last grab, [grab[0], grab[1]), kept in the claim
   // ring's spare LDS words (SGPRs are all in use)
@@ -996,12 +958,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   if (lane == 0) grab[0] = grab[1] = 0;
   __syncthreads();
   auto take_claim = [&]() -> uint32_t {
-    if (LB) {
-      uint32_t tk = 0;
-      if (lane == 0) tk = atomicAdd(&a.ctr->ticket, 1u);
-      const uint32_t t = __shfl(tk, 0, 64);
-      return t < a.ntiles ? t : ~0u;
-    }
     const uint32_t cur = grab[0], end = grab[1];
     if (cur < end) {
       if (lane == 0) grab[0] = cur + 1;
@@ -1128,7 +1084,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       }
     }
     lines_total += total;
-    if (LB) c.base = (uint32_t)tile_lookback(a.tile_state, t, total);
+    if (LB) c.base = (uint32_t)a.tile_state[t];  // the pre-pass's exclusive prefix
     else if (lane == 0) a.tile_state[t] = total;
     if (total) {
       if (lane == 0) {
@@ -1182,8 +1138,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       if (!pending && !tiles_left && !pf) break;
       continue;
     }
-    // aggregate queries: start copying the next claim so it lands during the round
-    if (!LB && !pf && !pending && tiles_left && cseq - hseq < CRING - 1u) {
+    // start copying the next claim so it lands during the round
+    if (!pf && !pending && tiles_left && cseq - hseq < CRING - 1u) {
       compact();
       if (a.arena - used >= need) {
         const uint32_t t = take_claim();

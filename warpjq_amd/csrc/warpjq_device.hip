@@ -345,6 +345,91 @@ __global__ __launch_bounds__(1024) void k_res_prefix(unsigned long long *cnt, ui
   }
 }
 
+// Line starts per claim for row outputs, by the rule k_main's enqueue
+// applies: q in [max(t0, start), min(t0 + claim, vend)) with q == start or
+// byte q - 1 == '\n'. One wave per claim, 16 bytes per lane per step.
+__global__ __launch_bounds__(256) void k_line_count(const Span *spans, uint32_t n_spans, uint32_t ntiles, uint32_t claim,
+                                                     unsigned long long *cnt) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < ntiles; t += nw) {
+    const uint32_t t0 = t * claim;
+    const Span sp = spans[span_of(spans, n_spans, t0)];
+    const uint32_t lo = max(t0, sp.start), hi = min(t0 + claim, sp.vend);
+    unsigned long long n = 0;
+    if (lo < hi) {
+      const uint8_t *d = sp.base - sp.vbase;           // d[q]: the byte at q position q
+      const uint32_t a0 = lo > sp.start ? lo - 1 : sp.start;  // newlines in [a0, a1) start a line
+      const uint32_t a1 = hi - 1;
+      uint32_t m32 = 0;
+      for (uint32_t b = (a0 >> 4) + lane; (b << 4) < a1; b += 64) {
+        const uint32_t p = b << 4;
+        uint32_t m = nl_mask16(*(const uint4 *)(d + p));
+        if (p < a0) m &= 0xFFFFu << (a0 - p);
+        if (p + 16u > a1) m &= (1u << (a1 - p)) - 1u;
+        m32 += __popc(m);
+      }
+      n = wave_sum((unsigned long long)m32) + (lo == sp.start ? 1u : 0u);
+    }
+    if (lane == 0) cnt[t] = n;
+  }
+}
+
+// Exclusive prefix of cnt[0..n) in place, three steps: per-1024 scans with
+// block totals, a scan of the totals, the totals added back.
+__global__ __launch_bounds__(1024) void k_tile_scan_blocks(unsigned long long *cnt, uint32_t n,
+                                                            unsigned long long *bsum) {
+  __shared__ unsigned long long wsum[16];
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t i = blockIdx.x * 1024 + threadIdx.x;
+  const unsigned long long v = i < n ? cnt[i] : 0;
+  unsigned long long x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned long long y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  unsigned long long p = 0;
+  for (uint32_t q = 0; q < w; q++) p += wsum[q];
+  if (i < n) cnt[i] = p + x - v;
+  if (threadIdx.x == 1023) bsum[blockIdx.x] = p + x;
+}
+
+__global__ __launch_bounds__(1024) void k_tile_scan_totals(unsigned long long *bsum, uint32_t nb) {
+  __shared__ unsigned long long wsum[16];
+  __shared__ unsigned long long run;
+  if (threadIdx.x == 0) run = 0;
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (uint32_t base = 0; base < nb; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const unsigned long long v = i < nb ? bsum[i] : 0;
+    unsigned long long x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned long long y = __shfl_up(x, o, 64);
+      if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    unsigned long long p = run;
+    for (uint32_t q = 0; q < w; q++) p += wsum[q];
+    if (i < nb) bsum[i] = p + x - v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (uint32_t q = 0; q < 16; q++) run += wsum[q];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_tile_scan_add(unsigned long long *cnt, uint32_t n,
+                                                         const unsigned long long *bsum) {
+  const uint32_t i = blockIdx.x * 1024 + threadIdx.x;
+  if (i < n) cnt[i] += bsum[blockIdx.x];
+}
+
 __global__ void k_res_records(DevCtr *ctr, const unsigned long long *prefix, uint32_t *fb_idx, const uint32_t *fb_tile,
                               uint32_t fb_cap) {
   uint32_t n = min(ctr->fb_count, fb_cap);
@@ -405,6 +490,7 @@ struct Slot {
   Span *d_spans = nullptr, *h_spans = nullptr;
   warpjq_amd_span *h_pieces = nullptr;
   uint32_t *d_long = nullptr;  // deferred long lines, 3 words each, zero between launches
+  unsigned long long *d_tsum = nullptr;  // block totals of the line-count scan
   std::vector<GraphEntry> graphs;  // captured chains of resident submits
   uint64_t graph_tick = 0;
   bool keys_blob = false;  // group keys copied out (batched submits)
@@ -506,6 +592,7 @@ static void free_slot(Slot &s) {
   (void)hipFree(s.d_fb_tile);
   (void)hipFree(s.d_spans);
   (void)hipFree(s.d_long);
+  (void)hipFree(s.d_tsum);
   for (GraphEntry &g : s.graphs) (void)hipGraphExecDestroy(g.exec);
   s.graphs.clear();
   (void)hipHostFree(s.h_spans);
@@ -549,6 +636,7 @@ static warpjq_status alloc_slot(warpjq_ctx *ctx, Slot &s, char *err, size_t err_
   HIP_TRY(halloc(&s.h_fb_span, fc), "hipHostMalloc(fb)");
   HIP_TRY(dalloc(&s.d_spans, MAX_SPANS), "hipMalloc(spans)");
   HIP_TRY(dalloc(&s.d_long, (size_t)ctx->max_tiles * 3), "hipMalloc(long lines)");
+  HIP_TRY(dalloc(&s.d_tsum, (size_t)ctx->max_tiles / 1024 + 2), "hipMalloc(tile sums)");
   HIP_TRY(hipMemset(s.d_long, 0, (size_t)ctx->max_tiles * 12), "hipMemset(long lines)");
   HIP_TRY(halloc(&s.h_spans, MAX_SPANS), "hipHostMalloc(spans)");
   s.h_pieces = new (std::nothrow) warpjq_amd_span[MAX_SPANS];
@@ -835,6 +923,14 @@ static warpjq_status enqueue_chain(warpjq_ctx *ctx, Slot &s, uint32_t ns, uint32
     a.long_q = s.d_long;
     a.long_cap = ctx->max_tiles;
     uint32_t grid = std::min<uint32_t>(ctx->grid, ntiles);
+    if (ctx->rows) {  // each claim's first line index, so claims can go in any order
+      const uint32_t nb = (ntiles + 1023) / 1024;
+      hipLaunchKernelGGL(k_line_count, dim3(std::min<uint32_t>(4096, (ntiles + 3) / 4)), dim3(256), 0, st, s.d_spans, ns,
+                         ntiles, claim, s.d_tile_state);
+      hipLaunchKernelGGL(k_tile_scan_blocks, dim3(nb), dim3(1024), 0, st, s.d_tile_state, ntiles, s.d_tsum);
+      hipLaunchKernelGGL(k_tile_scan_totals, dim3(1), dim3(1024), 0, st, s.d_tsum, nb);
+      hipLaunchKernelGGL(k_tile_scan_add, dim3(nb), dim3(1024), 0, st, s.d_tile_state, ntiles, s.d_tsum);
+    }
     if (timed) HIP_TRY(hipEventRecord(s.evm0, st), "hipEventRecord");
     switch (ctx->host.ns) {
       case 1: launch_main_1(ctx->grouped, ctx->rows, grid, st, a); break;
