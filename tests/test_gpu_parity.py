@@ -119,3 +119,34 @@ def test_long_lines():
             assert gpu == cpu, (q, fmt)
             for k in ("lines_in", "malformed", "type_errors"):
                 assert gst[k] == cst[k], (q, k)
+
+
+def test_long_line_group_key_across_window():
+    # long lines walked by the whole wave: a group key that crosses the LDS
+    # window edge (18 KiB) and keys of every length around 32 bytes
+    lines = []
+    for i, pad in enumerate((18000, 18400, 18430, 36000, 50000)):
+        key = ("k%d-" % (i % 2)) + "z" * 1000
+        lines.append('{"pad":"%s","host":"%s","bytes":%d}' % ("p" * pad, key, i))
+    for n in range(28, 37):
+        lines.append('{"host":"%s","bytes":1}' % ("h" * n))
+        lines.append('{"host":"%s","bytes":2}' % ("h" * n))
+    data = ("\n".join(lines * 3) + "\n").encode()
+    for q in ("group_by(.host) | count", "group_by(.host) | sum(.bytes)"):
+        cpu, cst, gpu, gst = both(q, data, chunk=1 << 20)
+        assert gpu == cpu, q
+
+
+def test_long_line_string_ends_at_every_offset():
+    # a long line's string whose closing quote falls at each offset of the
+    # 1 KiB steps of the wave-wide scan and around the LDS window edge
+    for kl in (5, 40, 1000):
+        lines = []
+        for pad in list(range(18416, 18450)) + list(range(17392, 17420)) + [36860, 36870, 9000, 10240]:
+            key = "k" + "z" * (kl - 1)
+            lines.append('{"pad":"%s","host":"%s","bytes":%d}' % ("p" * pad, key, pad))
+        data = ("\n".join(lines) + "\n").encode()
+        for q in ("group_by(.host) | count", "sum(.bytes)", ".bytes", 'select(.host == "k") | count'):
+            cpu, cst, gpu, gst = both(q, data, chunk=1 << 22)
+            assert gpu == cpu, (kl, q)
+            assert gst["malformed"] == cst["malformed"] == 0

@@ -286,17 +286,28 @@ struct WinReader {
     return s == 0 ? x.x : s == 1 ? x.y : s == 2 ? x.z : x.w;
   }
   __device__ __forceinline__ uint32_t byte(uint32_t i) { return (dword(i >> 2) >> ((i & 3) * 8)) & 0xFFu; }
+  // Makes the window hold [a, b) (every lane takes part), so code that only
+  // some lanes run can read those bytes without a refill. False when the
+  // range is longer than the window.
+  __device__ __forceinline__ bool ensure(uint32_t a, uint32_t b) {
+    if (b <= a) return true;
+    const uint32_t b0 = a >> 4, b1 = (b - 1) >> 4;
+    if (b1 - b0 + 1 > wblk) return false;
+    if (b0 - wb >= wblk || b1 - wb >= wblk) fill(b0);
+    return true;
+  }
   // First position >= j holding a byte a string scan must look at: lane k
   // tests block (j >> 4) + k, 1 KiB per step, and the lowest flagged lane
   // gives the answer to all.
   __device__ __forceinline__ uint32_t next_special_wide(uint32_t j) {
     const uint32_t lane = __lane_id();
-    uint32_t b0 = j >> 4;
+    const uint32_t bj = j >> 4;
+    uint32_t b0 = bj;
     for (;;) {
       if (b0 - wb >= wblk || b0 + 64u - wb > wblk) fill(b0);
       Blk16 v = block(b0 + lane);
       uint32_t m = movemask16(str_special(v.x), str_special(v.y), str_special(v.z), str_special(v.w));
-      if (lane == 0) m &= 0xFFFFu << (j & 15u);
+      if (lane == 0 && b0 == bj) m &= 0xFFFFu << (j & 15u);  // bytes before j, first step only
       const uint64_t bal = __ballot(m != 0);
       if (bal) {
         const uint32_t L = (uint32_t)__builtin_ctzll(bal);
@@ -532,7 +543,28 @@ struct LGroup {
   unsigned int hash[LGROUP];
   unsigned long long count[LGROUP], numeric[LGROUP], ilo[LGROUP], ihi[LGROUP], mnb[LGROUP], mxb[LGROUP];
   double fsum[LGROUP], fcomp[LGROUP];
+  uint4 kb[LGROUP * 2];  // the key bytes of entries with short, escape-free keys
 };
+
+#define KEY_INLINE 32u
+
+__device__ __forceinline__ bool key_inline(const GKey &k) { return k.len <= KEY_INLINE && !k.esc; }
+
+// The key's first 32 bytes, zero past its end (dword reads of the line).
+template <class R>
+__device__ __forceinline__ void key_words(R &rd, const GKey &k, uint32_t w[8]) {
+  const uint32_t o = k.off, sh = (o & 3u) * 8u;
+  uint32_t d[9];
+#pragma unroll
+  for (uint32_t j = 0; j < 9; j++) d[j] = (j * 4u < k.len + (o & 3u)) ? rd.dword((o >> 2) + j) : 0u;
+#pragma unroll
+  for (uint32_t q = 0; q < 8; q++) {
+    uint32_t x = sh ? ((d[q] >> sh) | (d[q + 1] << (32u - sh))) : d[q];
+    if (q * 4u >= k.len) x = 0;
+    else if (q * 4u + 4u > k.len) x &= (1u << (8u * (k.len - q * 4u))) - 1u;
+    w[q] = x;
+  }
+}
 
 __device__ __forceinline__ void lds_acc_f64(double *s, double *c, double v) {
   double old = atomicAdd(s, v);
@@ -550,12 +582,19 @@ __device__ __forceinline__ void group_line(R &rd, const Span *spans, uint32_t n_
   unsigned long long pk = pack_key(k);
   uint32_t pos = h & (LGROUP - 1);
   int found = -1;
+  const bool kin = key_inline(k);
+  uint32_t kw[8];
+  if (kin) key_words(rd, k, kw);
   for (uint32_t probe = 0; probe < 8; probe++) {
     unsigned long long cur = L.key[pos];
     if (cur == EMPTY_KEY) {
       unsigned long long old = atomicCAS(&L.key[pos], EMPTY_KEY, pk);
       if (old == EMPTY_KEY) {
         L.hash[pos] = h;
+        if (kin) {
+          L.kb[2 * pos] = make_uint4(kw[0], kw[1], kw[2], kw[3]);
+          L.kb[2 * pos + 1] = make_uint4(kw[4], kw[5], kw[6], kw[7]);
+        }
         found = (int)pos;
         break;
       }
@@ -566,8 +605,14 @@ __device__ __forceinline__ void group_line(R &rd, const Span *spans, uint32_t n_
       bool same = cur == pk;
       if (!same) {
         const GKey o = unpack_key(cur);
-        DevReader gr = span_reader(spans, n_spans, o.off);
-        same = group_eq(rd, k, gr, o);
+        if (kin && key_inline(o)) {  // both keys' bytes at hand: compare in LDS
+          const uint4 x = L.kb[2 * pos], y = L.kb[2 * pos + 1];
+          same = o.kind == k.kind && o.len == k.len && x.x == kw[0] && x.y == kw[1] && x.z == kw[2] &&
+                 x.w == kw[3] && y.x == kw[4] && y.y == kw[5] && y.z == kw[6] && y.w == kw[7];
+        } else {
+          DevReader gr = span_reader(spans, n_spans, o.off);
+          same = group_eq(rd, k, gr, o);
+        }
       }
       if (same) {
         found = (int)pos;
@@ -704,6 +749,10 @@ __device__ __forceinline__ void process_line(R &rd, const KArgs &a, const LProg 
       gk = group_key(rd, pick<NS>(sl, P.group_slot));
       if (gk.len > 0x7FFFF) undecided = true;
       else gh = group_hash(rd, gk);
+      // the long-line phase: lane 0 alone re-reads the key in group_line
+      if constexpr (wide_reader<R>::value) {
+        if (!undecided && !rd.ensure(gk.off, gk.off + gk.len)) undecided = true;
+      }
     }
     if (undecided) {
       r.status = ST_FALLBACK;
