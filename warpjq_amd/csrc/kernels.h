@@ -79,7 +79,7 @@ using namespace wjqd;
 #define ARENA_MAX 32768u             // offsets fit 15 bits; bit 15 marks unstaged lines
 #define QCAP 128u                    // per-wave LDS queue of line starts
 #define QMASK (QCAP - 1u)
-#define LGROUP 64u                   // per-wave LDS group table entries
+#define LGROUP 32u                   // per-wave LDS group table entries
 #define GROUP_MAX_PROBE 128u
 #define MIN_LINE_BYTES 24u           // kernels.cu:1984-1988 sizing rule
 #define DATA_PAD 256u
@@ -543,22 +543,22 @@ struct LGroup {
   unsigned int hash[LGROUP];
   unsigned long long count[LGROUP], numeric[LGROUP], ilo[LGROUP], ihi[LGROUP], mnb[LGROUP], mxb[LGROUP];
   double fsum[LGROUP], fcomp[LGROUP];
-  uint4 kb[LGROUP * 2];  // the key bytes of entries with short, escape-free keys
+  uint4 kb[LGROUP * 3];  // the key bytes of entries with short, escape-free keys
 };
 
-#define KEY_INLINE 32u
+#define KEY_INLINE 48u
 
 __device__ __forceinline__ bool key_inline(const GKey &k) { return k.len <= KEY_INLINE && !k.esc; }
 
-// The key's first 32 bytes, zero past its end (dword reads of the line).
+// The key's first 48 bytes, zero past its end (dword reads of the line).
 template <class R>
-__device__ __forceinline__ void key_words(R &rd, const GKey &k, uint32_t w[8]) {
+__device__ __forceinline__ void key_words(R &rd, const GKey &k, uint32_t w[12]) {
   const uint32_t o = k.off, sh = (o & 3u) * 8u;
-  uint32_t d[9];
+  uint32_t d[13];
 #pragma unroll
-  for (uint32_t j = 0; j < 9; j++) d[j] = (j * 4u < k.len + (o & 3u)) ? rd.dword((o >> 2) + j) : 0u;
+  for (uint32_t j = 0; j < 13; j++) d[j] = (j * 4u < k.len + (o & 3u)) ? rd.dword((o >> 2) + j) : 0u;
 #pragma unroll
-  for (uint32_t q = 0; q < 8; q++) {
+  for (uint32_t q = 0; q < 12; q++) {
     uint32_t x = sh ? ((d[q] >> sh) | (d[q + 1] << (32u - sh))) : d[q];
     if (q * 4u >= k.len) x = 0;
     else if (q * 4u + 4u > k.len) x &= (1u << (8u * (k.len - q * 4u))) - 1u;
@@ -583,7 +583,7 @@ __device__ __forceinline__ void group_line(R &rd, const Span *spans, uint32_t n_
   uint32_t pos = h & (LGROUP - 1);
   int found = -1;
   const bool kin = key_inline(k);
-  uint32_t kw[8];
+  uint32_t kw[12];
   if (kin) key_words(rd, k, kw);
   for (uint32_t probe = 0; probe < 8; probe++) {
     unsigned long long cur = L.key[pos];
@@ -592,8 +592,9 @@ __device__ __forceinline__ void group_line(R &rd, const Span *spans, uint32_t n_
       if (old == EMPTY_KEY) {
         L.hash[pos] = h;
         if (kin) {
-          L.kb[2 * pos] = make_uint4(kw[0], kw[1], kw[2], kw[3]);
-          L.kb[2 * pos + 1] = make_uint4(kw[4], kw[5], kw[6], kw[7]);
+          L.kb[3 * pos] = make_uint4(kw[0], kw[1], kw[2], kw[3]);
+          L.kb[3 * pos + 1] = make_uint4(kw[4], kw[5], kw[6], kw[7]);
+          L.kb[3 * pos + 2] = make_uint4(kw[8], kw[9], kw[10], kw[11]);
         }
         found = (int)pos;
         break;
@@ -606,9 +607,10 @@ __device__ __forceinline__ void group_line(R &rd, const Span *spans, uint32_t n_
       if (!same) {
         const GKey o = unpack_key(cur);
         if (kin && key_inline(o)) {  // both keys' bytes at hand: compare in LDS
-          const uint4 x = L.kb[2 * pos], y = L.kb[2 * pos + 1];
+          const uint4 x = L.kb[3 * pos], y = L.kb[3 * pos + 1], z = L.kb[3 * pos + 2];
           same = o.kind == k.kind && o.len == k.len && x.x == kw[0] && x.y == kw[1] && x.z == kw[2] &&
-                 x.w == kw[3] && y.x == kw[4] && y.y == kw[5] && y.z == kw[6] && y.w == kw[7];
+                 x.w == kw[3] && y.x == kw[4] && y.y == kw[5] && y.z == kw[6] && y.w == kw[7] &&
+                 z.x == kw[8] && z.y == kw[9] && z.z == kw[10] && z.w == kw[11];
         } else {
           DevReader gr = span_reader(spans, n_spans, o.off);
           same = group_eq(rd, k, gr, o);
