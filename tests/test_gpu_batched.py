@@ -202,6 +202,24 @@ def test_submit_spans_claim_sizes():
             assert r.n_lines == len(lines), claim
         finally:
             ctx.close()
+    # row outputs: line indices come from the line-count pre-pass, whatever
+    # the claim size and however the spans cut the lines
+    rq = "select(.status == 500) | {p: .path, b: .bytes}"
+    results = []
+    for claim in (1024, 3072, 8192):
+        ctx = w.Context(rq, chunk_bytes=1 << 22)
+        try:
+            ctx.set_claim(claim)
+            ctx.submit_spans(0, spans)
+            r = ctx.wait(0)
+            d = w.result_dict(r)
+            assert d["n_selected"] == want and d["n_lines"] == len(lines), claim
+            results.append((d["out"], d["line_idx"], d["row_off"]))
+        finally:
+            ctx.close()
+    assert results[0] == results[1] == results[2]
+    sel = [k for k, ln in enumerate(lines) if b'"status":500' in ln]
+    assert results[0][1] == sel
     del keep
 
 
