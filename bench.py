@@ -397,8 +397,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
-              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print("bench.py: runs on one GPU only; several GPUs would each take a share of the chunks "
+              "(or files / datasets) with one process per GPU, which is not built", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -104,7 +104,7 @@ struct DevCtr {
   unsigned long long n_lines, n_blank, n_invalid, n_type_error, n_fallback, n_sel, out_total;
   unsigned long long first_bad;  // (claim << 32 | rank) of the first invalid line, aggregate queries
   unsigned int first_invalid, overflow, saw_non_numeric, group_overflow;
-  unsigned int fb_count, n_groups, ticket, tiles_done;
+  unsigned int fb_count, n_groups;
   unsigned int rticket[NREG * RSTRIDE];  // one claim ticket per region, 128 B apart
   unsigned int blob_used;   // bytes of group keys copied out (multi-span submits)
   unsigned int long_count, long_next;  // deferred long lines: added, taken

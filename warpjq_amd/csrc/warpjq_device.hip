@@ -16,7 +16,7 @@ __global__ void k_reset(DevCtr *ctr, unsigned long long *tile_state, uint32_t nt
     ctr->first_invalid = 0xFFFFFFFFu;
     ctr->first_bad = ~0ull;
     ctr->overflow = ctr->saw_non_numeric = ctr->group_overflow = 0;
-    ctr->fb_count = ctr->n_groups = ctr->ticket = ctr->tiles_done = 0;
+    ctr->fb_count = ctr->n_groups = 0;
 
     ctr->blob_used = 0;
     ctr->long_count = ctr->long_next = 0;
