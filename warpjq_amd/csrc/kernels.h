@@ -1378,6 +1378,22 @@ __global__ __launch_bounds__(256) void k_emit(const Span *spans, uint32_t n_span
   const LProg P = stage_prog(P0, tab, tab_bytes, (LDS_AS uint8_t *)wjq_dyn_lds);
   if (ctr->overflow) return;
   uint64_t nsel = ctr->n_sel;
+  if (P.output_kind == WARPJQ_OUT_PASSTHROUGH && !P.csv_mode) {
+    // a whole-line row is the line's bytes up to a final '\r', then '\n':
+    // one wave copies a row, 64 contiguous bytes per step, no re-walk
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; r < nsel; r += nw) {
+      const uint32_t s = sel_start[sel_idx[r]];
+      const uint64_t o = row_off[r];
+      const uint32_t len = (uint32_t)(row_off[r + 1] - o) - 1u;
+      const Span sp = spans[span_of(spans, n_spans, s)];
+      const uint8_t *src = sp.base + (s - sp.vbase);
+      for (uint32_t k = lane; k < len; k += 64) out[o + k] = src[k];
+      if (lane == 0) out[o + len] = '\n';
+    }
+    return;
+  }
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nsel;
        r += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t idx = sel_idx[r];
