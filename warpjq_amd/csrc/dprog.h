@@ -107,6 +107,14 @@ inline bool build_dprog(const warpjq_program &p, DProgHost &h, std::string &err)
   d.n_slots = p.n_needed;
   d.n_cond = p.n_cond;
   d.fused_mask = fused;
+  uint32_t lens0 = 0;
+  for (uint32_t q = 0; q < p.n_needed; q++) {
+    if (!((fused >> q) & 1)) continue;
+    const warpjq_step &st = p.steps[h.spath[q].step_off];
+    if (st.kind == WARPJQ_STEP_KEY) lens0 |= 1u << (st.key_len < 31u ? st.key_len : 31u);
+    else lens0 |= 1u << 31;
+  }
+  d.key_lens0 = lens0;
   d.filter_mask = filter_mask;
   d.post_mask = all & ~filter_mask;
   d.has_filter = p.has_filter;

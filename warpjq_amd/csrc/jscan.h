@@ -76,6 +76,7 @@ struct DProg {
   uint32_t n_slots;
   uint32_t n_cond;
   uint32_t fused_mask;   // slots captured by the walk
+  uint32_t key_lens0;    // bit n: a fused slot's first step is a key of n bytes (bit 31: 31+ bytes, or no key)
   uint32_t filter_mask;  // slots the filter reads
   uint32_t post_mask;    // every other needed slot
   uint32_t has_filter;
@@ -723,7 +724,10 @@ WJQ_HD int walk(R &rd, uint32_t i, const PP &P, SlotV (&sl)[NS], uint32_t &te, u
       const uint32_t A = L < 4 ? lvl_get(act, L) : 0u;
       hit = 0;
       hit_fin = 0;
-      if (A) key_hits(rd, P, A, L, ks, i - 1, esc, hit, hit_fin);
+      // top-level keys of no wanted length skip the table lookups
+      const uint32_t kl = i - 1 - ks;
+      if (A && (L != 0 || esc || ((P.key_lens0 >> (kl < 31u ? kl : 31u)) & 1u)))
+        key_hits(rd, P, A, L, ks, i - 1, esc, hit, hit_fin);
       c = rd.byte(i);
       if (is_jws(c)) {
         i = skip_ws(rd, i);

@@ -332,7 +332,8 @@ struct LProg {
   const LDS_AS uint32_t *field_slot;
   const LDS_AS uint32_t *prefix_off;
   const LDS_AS uint32_t *prefix_len;
-  uint32_t n_slots, n_cond, fused_mask, filter_mask, post_mask, has_filter, output_kind, output_slot, agg_kind,
+  uint32_t n_slots, n_cond, fused_mask, key_lens0, filter_mask, post_mask, has_filter, output_kind, output_slot,
+      agg_kind,
       agg_slot, group_slot, n_fields, suffix_off, suffix_len, csv_mode;
 };
 
@@ -357,6 +358,7 @@ __device__ __forceinline__ LProg stage_prog(const DProg &P, const uint8_t *tab, 
   L.n_slots = P.n_slots;
   L.n_cond = P.n_cond;
   L.fused_mask = P.fused_mask;
+  L.key_lens0 = P.key_lens0;
   L.filter_mask = P.filter_mask;
   L.post_mask = P.post_mask;
   L.has_filter = P.has_filter;
