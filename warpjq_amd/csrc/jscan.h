@@ -166,12 +166,14 @@ struct wide_reader<R, std::void_t<decltype(&R::next_special_wide)>> : std::true_
 template <class R>
 WJQ_HD uint32_t next_special(R &rd, uint32_t j) {
   if constexpr (wide_reader<R>::value) return rd.next_special_wide(j);
+  // two blocks per step: the second read does not wait for the first
   for (;;) {
-    Blk16 v = rd.block(j >> 4);
-    uint32_t m = movemask16(str_special(v.x), str_special(v.y), str_special(v.z), str_special(v.w)) &
-                 (0xFFFFu << (j & 15u));
+    const Blk16 v = rd.block(j >> 4), u = rd.block((j >> 4) + 1u);
+    const uint32_t m = (movemask16(str_special(v.x), str_special(v.y), str_special(v.z), str_special(v.w)) |
+                        (movemask16(str_special(u.x), str_special(u.y), str_special(u.z), str_special(u.w)) << 16)) &
+                       (0xFFFFFFFFu << (j & 15u));
     if (m) return (j & ~15u) + (uint32_t)ctz32(m);
-    j = (j | 15u) + 1u;
+    j = (j | 15u) + 17u;
   }
 }
 
