@@ -180,11 +180,13 @@ WJQ_HD uint32_t next_special(R &rd, uint32_t j) {
 // First position >= j whose byte is not an ASCII digit.
 template <class R>
 WJQ_HD uint32_t digits_end(R &rd, uint32_t j) {
-  for (;;) {
-    Blk16 v = rd.block(j >> 4);
-    uint32_t m = movemask16(non_digit(v.x), non_digit(v.y), non_digit(v.z), non_digit(v.w)) & (0xFFFFu << (j & 15u));
+  for (;;) {  // two blocks per step, as next_special
+    const Blk16 v = rd.block(j >> 4), u = rd.block((j >> 4) + 1u);
+    const uint32_t m = (movemask16(non_digit(v.x), non_digit(v.y), non_digit(v.z), non_digit(v.w)) |
+                        (movemask16(non_digit(u.x), non_digit(u.y), non_digit(u.z), non_digit(u.w)) << 16)) &
+                       (0xFFFFFFFFu << (j & 15u));
     if (m) return (j & ~15u) + (uint32_t)ctz32(m);
-    j = (j | 15u) + 1u;
+    j = (j | 15u) + 17u;
   }
 }
 
