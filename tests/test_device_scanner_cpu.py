@@ -151,3 +151,58 @@ def test_number_parser_hard_cases():
     assert hostsim.parse_number(b"-0")[1] and hostsim.parse_number(b"9007199254740992")[1]
     assert not hostsim.parse_number(b"9007199254740993")[1]  # past 2^53: not exact as an integer
     assert not hostsim.parse_number(b"1.0")[1]
+
+
+def _nested_filter(kind, n):
+    """A filter nesting n comparisons to the right: a op (b op (c op ...))."""
+    terms = [f".a == {k + 1}" for k in range(n)]
+    if kind == "not":
+        terms = [f"(.a == {k + 1} | not)" if k % 3 == 1 else t for k, t in enumerate(terms)]
+    op = {"or": "or", "and": "and", "not": "or", "mix": None}[kind]
+    q = terms[-1]
+    for k in range(n - 2, -1, -1):
+        o = op or ("and" if k % 2 else "or")
+        q = f"{terms[k]} {o} ({q})"
+    return f"select({q}) | count"
+
+
+@pytest.mark.parametrize("kind", ["or", "and", "not", "mix"])
+def test_deep_filters_match_oracle(kind):
+    # The RPN of a right-nested filter holds one value per level; past 32 or
+    # 64 values a narrow device stack would wrap. Depths 1..65 (64 levels of
+    # parentheses, the parser's limit) against the oracle on lines that
+    # select at either end of the chain and in the middle.
+    lines = [b'{"a":1}', b'{"a":33}', b'{"a":34}', b'{"a":64}', b'{"a":65}', b'{"a":0}', b'{"b":1}', b'{"a":null}']
+    for n in (1, 2, 31, 32, 33, 34, 63, 64, 65):
+        if kind == "not" and n == 65:
+            continue  # a negated term inside the innermost group is a 65th level
+        q = _nested_filter(kind, n)
+        ev = hostsim.Evaluator(q)
+        for ln in lines:
+            out, st = oracle.run(q, ln + b"\n", threads=1)
+            o, _, _ = ev.eval(ln)
+            assert o.status == 0, (n, q, ln)
+            assert int(o.selected) == int(out), (kind, n, ln)
+
+
+def test_rpn_reordering_keeps_every_value():
+    # Sethi-Ullman reordering only swaps and/or operands; truth tables of
+    # random small filters over three booleans agree with the oracle.
+    rng = random.Random(5)
+
+    def gen(d):
+        if d == 0 or rng.random() < 0.3:
+            return rng.choice([".x", ".y", ".z", ".x == true", ".y == false"])
+        k = rng.randrange(3)
+        if k == 0:
+            return f"({gen(d - 1)} | not)"
+        return f"({gen(d - 1)} {'and' if k == 1 else 'or'} {gen(d - 1)})"
+
+    rows = [b'{"x":%s,"y":%s,"z":%s}' % (a, b, c) for a in (b"true", b"false") for b in (b"true", b"false")
+            for c in (b"true", b"null")]
+    for _ in range(60):
+        q = f"select({gen(5)}) | count"
+        ev = hostsim.Evaluator(q)
+        for ln in rows:
+            out, _ = oracle.run(q, ln + b"\n", threads=1)
+            assert int(ev.eval(ln)[0].selected) == int(out), (q, ln)

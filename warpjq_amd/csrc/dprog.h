@@ -12,9 +12,102 @@ namespace wjqd {
 struct DProgHost {
   std::vector<SlotPath> spath;
   std::vector<uint32_t> field_slot;
+  std::vector<warpjq_cond_op> cond;  // the filter RPN the kernels run (see order_rpn)
   DProg d{};  // host pointers into the caller's warpjq_program and into this
   uint32_t ns = 1;  // kernel template width: 1, 2, 4, 8 or 16 slots
 };
+
+// The kernels keep the filter's boolean stack in one 64-bit word
+// (eval_cond), so the RPN they run must never hold more than 64 values.
+#define WJQ_COND_STACK 64u
+
+// Replays an RPN filter: false when it is malformed (an operator without
+// its operands, or not exactly one value left); `depth` gets the largest
+// stack it reaches.
+inline bool rpn_depth(const warpjq_cond_op *c, uint32_t n, uint32_t &depth) {
+  uint32_t sp = 0;
+  depth = 0;
+  for (uint32_t k = 0; k < n; k++) {
+    switch (c[k].op) {
+      case WARPJQ_COND_CMP:
+      case WARPJQ_COND_TRUTHY: sp++; break;
+      case WARPJQ_COND_NOT:
+        if (sp < 1) return false;
+        break;
+      case WARPJQ_COND_AND:
+      case WARPJQ_COND_OR:
+        if (sp < 2) return false;
+        sp--;
+        break;
+      default: return false;
+    }
+    if (sp > depth) depth = sp;
+  }
+  return n == 0 || sp == 1;
+}
+
+// Re-emits a well-formed RPN with the deeper operand of every and/or first
+// (Sethi-Ullman order), so the stack grows with the logarithm of the number
+// of operands rather than with their nesting. The kernels evaluate every
+// operand (no short circuit) and and/or are commutative, so the value of the
+// filter for every line is unchanged.
+inline void order_rpn(const warpjq_cond_op *c, uint32_t n, std::vector<warpjq_cond_op> &out) {
+  struct Node {
+    warpjq_cond_op op;
+    int32_t a, b;
+    uint32_t need;  // stack slots its evaluation takes
+  };
+  std::vector<Node> nodes;
+  std::vector<int32_t> st;
+  nodes.reserve(n);
+  for (uint32_t k = 0; k < n; k++) {
+    Node x{c[k], -1, -1, 1};
+    if (c[k].op == WARPJQ_COND_NOT) {
+      x.a = st.back();
+      st.pop_back();
+      x.need = nodes[x.a].need;
+    } else if (c[k].op == WARPJQ_COND_AND || c[k].op == WARPJQ_COND_OR) {
+      x.b = st.back();
+      st.pop_back();
+      x.a = st.back();
+      st.pop_back();
+      const uint32_t na = nodes[x.a].need, nb = nodes[x.b].need;
+      x.need = na == nb ? na + 1 : (na > nb ? na : nb);
+    }
+    nodes.push_back(x);
+    st.push_back((int32_t)nodes.size() - 1);
+  }
+  out.clear();
+  if (st.empty()) return;
+  // iterative post-order, deeper child first
+  std::vector<std::pair<int32_t, int>> work{{st.back(), 0}};
+  while (!work.empty()) {
+    auto &[id, stage] = work.back();
+    const Node &x = nodes[id];
+    if (x.a < 0) {
+      out.push_back(x.op);
+      work.pop_back();
+    } else if (x.b < 0) {
+      if (stage == 0) {
+        stage = 1;
+        work.push_back({x.a, 0});
+      } else {
+        out.push_back(x.op);
+        work.pop_back();
+      }
+    } else {
+      const bool b_first = nodes[x.b].need > nodes[x.a].need;
+      if (stage < 2) {
+        const int32_t next = (stage == 0) == b_first ? x.b : x.a;
+        stage++;
+        work.push_back({next, 0});
+      } else {
+        out.push_back(x.op);
+        work.pop_back();
+      }
+    }
+  }
+}
 
 // Returns false (with a message) for a program the kernels cannot host.
 inline bool build_dprog(const warpjq_program &p, DProgHost &h, std::string &err) {
@@ -84,6 +177,20 @@ inline bool build_dprog(const warpjq_program &p, DProgHost &h, std::string &err)
     }
     filter_mask |= 1u << s;
   }
+  uint32_t depth = 0;
+  if (!rpn_depth(p.cond_rpn, p.n_cond, depth)) {
+    err = "malformed filter program";
+    return false;
+  }
+  if (depth <= WJQ_COND_STACK) {
+    h.cond.assign(p.cond_rpn, p.cond_rpn + p.n_cond);
+  } else {
+    order_rpn(p.cond_rpn, p.n_cond, h.cond);
+    if (!rpn_depth(h.cond.data(), (uint32_t)h.cond.size(), depth) || depth > WJQ_COND_STACK) {
+      err = "filter needs a deeper stack than the kernels keep";
+      return false;
+    }
+  }
   uint32_t all = p.n_needed == 32 ? ~0u : ((1u << p.n_needed) - 1);
   h.field_slot.assign(p.n_fields, 0);
   for (uint32_t f = 0; f < p.n_fields; f++) {
@@ -97,7 +204,7 @@ inline bool build_dprog(const warpjq_program &p, DProgHost &h, std::string &err)
   DProg &d = h.d;
   d.steps = p.steps;
   d.cmps = p.cmps;
-  d.cond = p.cond_rpn;
+  d.cond = h.cond.data();
   d.blob = p.blob;
   d.spath = h.spath.data();
   d.slot_of_path = p.slot_of_path;

@@ -1065,10 +1065,13 @@ WJQ_HD bool truthy(R &rd, const SlotV &s) {
   return !(s.kind == JK_BOOL && rd.byte(s.off) == 'f');
 }
 
-// Evaluates the RPN filter. Returns 0/1, or 2 when undecided.
+// Evaluates the RPN filter. Returns 0/1, or 2 when undecided. The stack is
+// one 64-bit word: build_dprog admits only programs whose RPN never holds
+// more than 64 values (reordering deep ones), so `sp` stays in 0..64.
 template <int NS, class R, class PP>
 WJQ_HD int eval_cond(R &rd, const PP &P, const SlotV (&sl)[NS]) {
-  uint32_t stack = 0, sp = 0;
+  uint64_t stack = 0;
+  uint32_t sp = 0;
   for (uint32_t k = 0; k < P.n_cond; k++) {
     const auto &o = P.cond[k];
     uint32_t v;
@@ -1090,14 +1093,14 @@ WJQ_HD int eval_cond(R &rd, const PP &P, const SlotV (&sl)[NS]) {
         if ((uint32_t)q == s) x = sl[q];
       v = truthy(rd, x) ? 1u : 0u;
     } else if (o.op == WARPJQ_COND_NOT) {
-      stack ^= 1u << (sp - 1);
+      stack ^= 1ull << (sp - 1);
       continue;
     } else {
-      uint32_t b = (stack >> (sp - 1)) & 1, a = (stack >> (sp - 2)) & 1;
+      uint32_t b = (uint32_t)(stack >> (sp - 1)) & 1, a = (uint32_t)(stack >> (sp - 2)) & 1;
       sp -= 2;
       v = o.op == WARPJQ_COND_AND ? (a & b) : (a | b);
     }
-    stack = (stack & ~(1u << sp)) | (v << sp);
+    stack = (stack & ~(1ull << sp)) | ((uint64_t)v << sp);
     sp++;
   }
   return (int)(stack & 1);

@@ -757,7 +757,7 @@ static warpjq_status put_program(warpjq_ctx *ctx, const warpjq_program *prog, ch
   const DProg &h = ctx->host.d;
   size_t o_steps = put_table(buf, prog->steps, prog->n_steps);
   size_t o_cmps = put_table(buf, prog->cmps, prog->n_cmps);
-  size_t o_cond = put_table(buf, prog->cond_rpn, prog->n_cond);
+  size_t o_cond = put_table(buf, ctx->host.cond.data(), ctx->host.cond.size());
   size_t o_blob = put_table(buf, prog->blob, prog->n_blob);
   size_t o_spath = put_table(buf, ctx->host.spath.data(), ctx->host.spath.size());
   size_t o_sop = put_table(buf, prog->slot_of_path, prog->n_paths);
