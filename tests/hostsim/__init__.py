@@ -31,6 +31,12 @@ def lib():
         L.hs_parse_number.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64)]
         L.hs_parse_number.restype = ctypes.c_int
+        L.hs_parse_number_exact.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+        L.hs_parse_number_exact.restype = ctypes.c_int
+        L.hs_slow_decimal.argtypes = [ctypes.c_char_p, ctypes.c_uint64]
+        L.hs_slow_decimal.restype = ctypes.c_double
+        L.hs_validate_deep.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32]
+        L.hs_validate_deep.restype = ctypes.c_int
         L.hs_validate.argtypes = [ctypes.c_char_p, ctypes.c_uint64]
         L.hs_validate.restype = ctypes.c_int
         _lib = L
@@ -65,6 +71,20 @@ def parse_number(tok):
     iv = ctypes.c_int64()
     ok = lib().hs_parse_number(tok, len(tok), ctypes.byref(v), ctypes.byref(ii), ctypes.byref(iv))
     return (v.value, bool(ii.value), iv.value) if ok else None
+
+
+def parse_number_exact(tok):
+    v = ctypes.c_double()
+    ok = lib().hs_parse_number_exact(tok, len(tok), ctypes.byref(v))
+    return v.value if ok else None
+
+
+def slow_decimal(tok):
+    return lib().hs_slow_decimal(tok, len(tok))
+
+
+def validate_deep(line, cap=24576):
+    return lib().hs_validate_deep(line, len(line), cap)
 
 
 def validate(line):

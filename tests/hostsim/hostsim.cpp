@@ -157,6 +157,68 @@ int hs_parse_number(const uint8_t *tok, uint64_t n, double *v, int *is_int, int6
   return 1;
 }
 
+// The same with a digit buffer, as the wave-wide phase reads lines: every
+// number is decided.
+struct ExactReader : HostReader {
+  uint8_t *buf;
+  uint8_t *dec_buf() { return buf; }
+};
+
+int hs_parse_number_exact(const uint8_t *tok, uint64_t n, double *v) {
+  std::vector<uint8_t> b(tok, tok + n);
+  b.push_back('\n');
+  std::vector<uint8_t> digits(WJQ_DEC_BUF);
+  ExactReader rd;
+  rd.d = b.data();
+  rd.n = (uint32_t)b.size();
+  rd.buf = digits.data();
+  NumVal nv;
+  if (!parse_number(rd, 0, (uint32_t)n, nv)) return 0;
+  *v = nv.v;
+  return 1;
+}
+
+// The exact path alone (no fast path in front of it).
+double hs_slow_decimal(const uint8_t *tok, uint64_t n) {
+  std::vector<uint8_t> b(tok, tok + n);
+  b.push_back('\n');
+  std::vector<uint8_t> digits(WJQ_DEC_BUF);
+  HostReader rd{b.data(), (uint32_t)b.size()};
+  uint64_t bits = slow_decimal(rd, 0, (uint32_t)n, digits.data());
+  double d;
+  memcpy(&d, &bits, 8);
+  return d;
+}
+
+// A reader with a deep container stack and a digit buffer, as the
+// wave-wide phase reads a line (levels past 64 in its LDS).
+struct DeepReader : ExactReader {
+  std::vector<uint8_t> *bits;
+  uint32_t cap;
+  uint32_t deep_cap() const { return cap; }
+  void deep_set(uint32_t d, bool a) { (*bits)[d] = a; }
+  bool deep_get(uint32_t d) const { return (*bits)[d] != 0; }
+};
+
+// walk() through the deep reader: status as hs_validate.
+int hs_validate_deep(const uint8_t *line, uint64_t n, uint32_t cap) {
+  std::vector<uint8_t> buf(line, line + n);
+  buf.push_back('\n');
+  std::vector<uint8_t> digits(WJQ_DEC_BUF), bits(cap + 1);
+  DeepReader rd;
+  rd.d = buf.data();
+  rd.n = (uint32_t)buf.size();
+  rd.buf = digits.data();
+  rd.bits = &bits;
+  rd.cap = cap;
+  DProg P{};
+  SlotV sl[1];
+  sl[0] = SlotV{0, 0, 0};
+  uint32_t te = 0, end = 0;
+  uint32_t i = skip_ws(rd, 0);
+  return walk<1>(rd, i, P, sl, te, end);
+}
+
 // The device's string scanner + structural validation on one line.
 int hs_validate(const uint8_t *line, uint64_t n) {
   std::vector<uint8_t> buf(line, line + n);

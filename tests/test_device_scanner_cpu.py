@@ -206,3 +206,74 @@ def test_rpn_reordering_keeps_every_value():
         for ln in rows:
             out, _ = oracle.run(q, ln + b"\n", threads=1)
             assert int(ev.eval(ln)[0].selected) == int(out), (q, ln)
+
+
+def _hard_numbers(rng):
+    """Numbers the 128-bit path tends to leave open: more than 19 digits near
+    rounding boundaries, halfway cases between adjacent doubles written out
+    in full, subnormals, and digit strings far past 768 digits."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 1200
+    out = []
+    for _ in range(300):
+        # exact midpoint of two adjacent doubles (its decimal expansion is finite)
+        b = rng.getrandbits(63) & ~(0x7FF << 52) | (rng.randrange(1, 0x7FE) << 52)
+        x = struct.unpack("<d", struct.pack("<Q", b))[0]
+        y = struct.unpack("<d", struct.pack("<Q", b + 1))[0]
+        mid = (Decimal(x) + Decimal(y)) / 2
+        s = format(mid, "f") if abs(mid.adjusted()) < 40 else format(mid, "e")
+        out.append(s)
+        # just above and below the midpoint, in many digits
+        d = Decimal(10) ** (mid.adjusted() - 800)
+        out.append(format(mid + d, "e"))
+        out.append(format(mid - d, "e"))
+    for _ in range(300):
+        out.append(str(rng.randrange(10 ** 19, 10 ** 40)) + "e" + str(rng.randrange(-360, 300)))
+        out.append("0." + "0" * rng.randrange(300, 330) + str(rng.randrange(1, 10 ** 25)))
+    out += ["2.4703282292062327208828439643411068618252990130716238221279284125033775363510437593264991818081799618989828234772285886546332835517796989819938739800539093906315035659515570226392290858392449105184435931802849936536152500319370457678249219365623669863658480757001585769269903706311928279558551332927834338409351978015531246597263579574622766465272827220056374006485499977096599470454020828166226237857393450736339007967761930577506740176324673600968951340535537458516661134223766678604162159680461914467291840300530057530849048765391711386591646239524912623653881879636239373280423891018672348497668235089863388587925628302755995657524455507255189313690836254779186948667994968324049705821028513185451396213837722826145437693412532098591327667236328125e-324",
+            "2.4703282292062327208828439643411068618252990130716238221279284125033775363510437593264991818081799618989828234772285886546332835517796989819938739800539093906315035659515570226392290858392449105184435931802849936536152500319370457678249219365623669863658480757001585769269903706311928279558551332927834338409351978015531246597263579574622766465272827220056374006485499977096599470454020828166226237857393450736339007967761930577506740176324673600968951340535537458516661134223766678604162159680461914467291840300530057530849048765391711386591646239524912623653881879636239373280423891018672348497668235089863388587925628302755995657524455507255189313690836254779186948667994968324049705821028513185451396213837722826145437693412532098591327667236328125001e-324",
+            "1" + "0" * 900, "0." + "0" * 400 + "1", "9" * 850 + "e-600", "1.7976931348623158079e308",
+            "1.7976931348623158080e308", "4.9406564584124654e-324", "2.2250738585072013830902327173324040642192159804623318306e-308"]
+    return out
+
+
+def test_exact_decimal_matches_strtod():
+    # The wave-wide phase settles every number with the simple decimal
+    # conversion; glibc strtod is correctly rounded (the contract of Rust's
+    # f64::from_str behind json.rs:73-78). The exact path is checked alone on
+    # every case, and behind the fast paths, with nothing left undecided.
+    rng = random.Random(23)
+    cases = _hard_numbers(rng) + [_gen(rng) for _ in range(20000)]
+    cases += ["9007199254740993", "2.2250738585072011e-308", "2.2250738585072012e-308", "4.9e-324",
+              "2.4703282292062327e-324", "2.4703282292062328e-324", "1.7976931348623157e308",
+              "1.7976931348623158e308", "1e400", "1e-400", "0.1", "-0", "123456789012345678901234567890",
+              "9007199254740992.5", "7.2057594037927933e16", "0.000000000000000000000000000001", "0", "-0.0e5"]
+    for s in cases:
+        if "inf" in s or "nan" in s:
+            continue
+        b = s.encode()
+        ref = struct.pack("<d", _libc.strtod(b, None))
+        assert struct.pack("<d", hostsim.slow_decimal(b)) == ref, s
+        v = hostsim.parse_number_exact(b)
+        assert v is not None and struct.pack("<d", v) == ref, s
+
+
+def test_deep_lines_walked_with_a_deep_stack():
+    # differential.rs:521-533: a 5,000-deep line is valid JSON and the CPU
+    # engine decides it; the per-lane walk declines it (64 levels in
+    # registers) and the wave-wide phase, whose stack lives in LDS, decides
+    # it like the oracle. Mismatched brackets deep down stay invalid.
+    deep = b"[" * 5000 + b"]" * 5000
+    assert hostsim.validate(deep) == 3
+    assert hostsim.validate_deep(deep) == 0
+    assert hostsim.validate_deep(b'{"a":' * 3000 + b"1" + b"}" * 3000) == 0
+    assert hostsim.validate_deep(b"[" * 5000 + b"]" * 4999 + b"}") == 1
+    assert hostsim.validate_deep(b"[" * 4000 + b"{" + b"]" * 4001) == 1
+    assert hostsim.validate_deep(b"[" * 5000 + b"]" * 4999) == 1
+    assert hostsim.validate_deep(b"[" * 100 + b"]" * 100, cap=100) == 0
+    assert hostsim.validate_deep(b"[" * 101 + b"]" * 101, cap=100) == 0  # the innermost empty array takes no level
+    assert hostsim.validate_deep(b"[" * 102 + b"]" * 102, cap=100) == 3  # past the LDS stack: declined
+    for n in (64, 65, 66, 200):
+        line = b'{"k":' + b"[" * n + b'{"x":1}' + b"]" * n + b"}"
+        out, st = oracle.run("count", line + b"\n", threads=1)
+        assert hostsim.validate_deep(line) == (0 if st["malformed"] == 0 else 1)

@@ -43,7 +43,9 @@ enum { ST_OK = 0, ST_INVALID = 1, ST_TYPE_ERROR = 2, ST_FALLBACK = 3, ST_BLANK =
 
 // Paths of at most this many steps are captured during the validating walk.
 #define WJQ_FUSED_STEPS 3
-// Depth of the container bit-stack; deeper lines are declined (FALLBACK).
+// Depth of the container bit-stack kept in registers; deeper levels need a
+// reader with a stack of its own (the wave-wide phase), else the line is
+// declined (FALLBACK) there and handed to that phase.
 #define WJQ_MAX_DEPTH 64
 #define WJQ_END 0x0Au  // '\n', the line terminator; also returned past the chunk
 
@@ -495,6 +497,221 @@ WJQ_HD uint64_t f64_to_bits(double d) {
   return x.u;
 }
 
+// ---------------------------------------------------------------------------
+// exact decimal -> binary64 for the numbers the 128-bit path cannot settle:
+// the simple decimal conversion (N. Tao, "The Simple Decimal Conversion
+// Algorithm", 2020; the method behind Go's strconv and Rust's dec2flt slow
+// path, whose correctly rounded result Rust's f64::from_str gives,
+// json.rs:73-78). The decimal is held as up to WJQ_DEC_DIGITS significant
+// digits (enough to decide any binary64: 767 + 1) plus a flag for nonzero
+// digits dropped past them, and scaled by powers of two until it lies in
+// [0.5, 1); its 53-bit mantissa is then rounded half to even. `B` is a byte
+// buffer of WJQ_DEC_BUF bytes (a wave's LDS on the device, all lanes
+// computing the same number in step).
+// ---------------------------------------------------------------------------
+
+#define WJQ_DEC_DIGITS 800
+#define WJQ_DEC_BUF (WJQ_DEC_DIGITS + 32)
+
+template <class B>
+struct Decimal {
+  B d;          // digit values 0..9
+  uint32_t nd;  // significant digits held
+  int32_t dp;   // value = 0.d[0]d[1]... x 10^dp
+  bool trunc;   // nonzero digits were dropped past WJQ_DEC_DIGITS
+
+  WJQ_HD void trim() {
+    while (nd > 0 && d[nd - 1] == 0) nd--;
+    if (nd == 0) dp = 0;
+  }
+  // x 2^-k, 1 <= k <= 60: long division from the most significant digit
+  WJQ_HD void shr(uint32_t k) {
+    uint32_t r = 0, w = 0;
+    uint64_t n = 0;
+    while ((n >> k) == 0) {
+      if (r >= nd) {
+        if (n == 0) {
+          nd = 0;
+          dp = 0;
+          return;
+        }
+        while ((n >> k) == 0) {
+          n *= 10;
+          r++;
+        }
+        break;
+      }
+      n = n * 10 + d[r];
+      r++;
+    }
+    dp -= (int32_t)r - 1;
+    const uint64_t mask = (1ull << k) - 1;
+    while (r < nd) {
+      const uint32_t q = (uint32_t)(n >> k);
+      n = ((n & mask) * 10) + d[r];
+      r++;
+      d[w++] = (uint8_t)q;
+    }
+    while (n > 0) {
+      const uint32_t q = (uint32_t)(n >> k);
+      n = (n & mask) * 10;
+      if (w < WJQ_DEC_DIGITS) d[w++] = (uint8_t)q;
+      else if (q) trunc = true;
+    }
+    nd = w;
+    trim();
+  }
+  // x 2^k, 1 <= k <= 60: from the least significant digit, into positions
+  // shifted right by an upper bound on the new leading digits, then moved back
+  WJQ_HD void shl(uint32_t k) {
+    const uint32_t D = ((k * 1233u) >> 12) + 1u;  // > k log10(2)
+    int32_t rd_ = (int32_t)nd - 1;
+    int32_t wr = (int32_t)nd - 1 + (int32_t)D;
+    uint64_t n = 0;
+    while (rd_ >= 0) {
+      n += (uint64_t)d[rd_] << k;
+      const uint64_t q = n / 10;
+      d[wr] = (uint8_t)(n - q * 10);
+      n = q;
+      wr--;
+      rd_--;
+    }
+    while (n > 0) {
+      const uint64_t q = n / 10;
+      d[wr] = (uint8_t)(n - q * 10);
+      n = q;
+      wr--;
+    }
+    const uint32_t s = (uint32_t)(wr + 1);
+    uint32_t total = nd + D - s;
+    for (uint32_t i = 0; i < total; i++) d[i] = d[i + s];
+    dp += (int32_t)(D - s);
+    if (total > WJQ_DEC_DIGITS) {
+      for (uint32_t i = WJQ_DEC_DIGITS; i < total; i++)
+        if (d[i]) trunc = true;
+      total = WJQ_DEC_DIGITS;
+    }
+    nd = total;
+    trim();
+  }
+  // nearest integer, ties to even (a dropped nonzero tail breaks a tie up)
+  WJQ_HD uint64_t round_int() const {
+    if (nd == 0 || dp < 0) return 0;
+    if (dp > 18) return ~0ull;
+    uint64_t n = 0;
+    for (int32_t i = 0; i < dp; i++) n = n * 10 + ((uint32_t)i < nd ? d[i] : 0u);
+    bool up = false;
+    if ((uint32_t)dp < nd) {
+      const uint32_t c = d[dp];
+      up = c >= 5;
+      if (c == 5 && (uint32_t)dp + 1 == nd) up = trunc || (n & 1);
+    }
+    return n + (up ? 1 : 0);
+  }
+};
+
+// floor(n log2(10)) for n < 19: a right (left) shift by it keeps a decimal
+// point of n (-n) from overshooting [0.5, 1).
+WJQ_HD uint32_t dec_shift_for(uint32_t n) {
+  const uint8_t t[19] = {0, 3, 6, 9, 13, 16, 19, 23, 26, 29, 33, 36, 39, 43, 46, 49, 53, 56, 59};
+  return n < 19 ? t[n] : 60u;
+}
+
+// Token [a, b) is a validated JSON number; its correctly rounded binary64
+// bits (sign included).
+template <class R, class B>
+WJQ_HD uint64_t slow_decimal(R &rd, uint32_t a, uint32_t b, B buf) {
+  Decimal<B> x{buf, 0, 0, false};
+  uint32_t i = a;
+  const bool neg = rd.byte(i) == '-';
+  if (neg) i++;
+  auto push = [&](uint32_t c) {
+    if (x.nd < WJQ_DEC_DIGITS) x.d[x.nd++] = (uint8_t)(c - '0');
+    else if (c != '0') x.trunc = true;
+  };
+  for (; i < b; i++) {
+    const uint32_t c = rd.byte(i);
+    if (!is_digit(c)) break;
+    if (x.nd == 0 && c == '0') continue;
+    push(c);
+    x.dp++;
+  }
+  if (i < b && rd.byte(i) == '.') {
+    for (i++; i < b; i++) {
+      const uint32_t c = rd.byte(i);
+      if (!is_digit(c)) break;
+      if (x.nd == 0 && c == '0') x.dp--;
+      else push(c);
+    }
+  }
+  int64_t e = 0;
+  if (i < b && (rd.byte(i) | 0x20u) == 'e') {
+    i++;
+    bool eneg = false;
+    if (rd.byte(i) == '+') i++;
+    else if (rd.byte(i) == '-') {
+      eneg = true;
+      i++;
+    }
+    for (; i < b; i++)
+      if (e < 100000) e = e * 10 + (rd.byte(i) - '0');
+    if (eneg) e = -e;
+  }
+  const uint64_t sign = neg ? (1ull << 63) : 0ull;
+  x.trim();
+  if (x.nd == 0) return sign;
+  const int64_t dp = (int64_t)x.dp + e;
+  if (dp >= 310) return sign | 0x7FF0000000000000ull;
+  if (dp <= -326) return sign;
+  x.dp = (int32_t)dp;
+  int32_t exp2 = 0;
+  while (x.dp > 0) {
+    const uint32_t n = dec_shift_for((uint32_t)x.dp);
+    x.shr(n);
+    exp2 += (int32_t)n;
+  }
+  while (x.dp < 0 || (x.dp == 0 && x.d[0] < 5)) {
+    uint32_t n;
+    if (x.dp == 0) n = x.d[0] >= 2 ? 1u : 2u;
+    else n = dec_shift_for((uint32_t)(-x.dp));
+    x.shl(n);
+    exp2 -= (int32_t)n;
+  }
+  // value = V x 2^exp2 with V in [0.5, 1), i.e. (2V) x 2^E
+  int32_t E = exp2 - 1;
+  if (E < -1022) {
+    uint32_t n = (uint32_t)(-1022 - E);
+    while (n > 0) {
+      const uint32_t k = n > 60 ? 60u : n;
+      x.shr(k);
+      n -= k;
+    }
+    E = -1022;
+  }
+  if (E > 1023) return sign | 0x7FF0000000000000ull;
+  x.shl(53);
+  uint64_t m = x.round_int();
+  if (m >= (1ull << 53)) {
+    x.shr(1);
+    E++;
+    m = x.round_int();
+    if (E > 1023) return sign | 0x7FF0000000000000ull;
+  }
+  uint64_t biased = 0;
+  if (m >= (1ull << 52)) {
+    biased = (uint64_t)(E + 1023);
+    m &= (1ull << 52) - 1;
+  }
+  return sign | (biased << 52) | m;
+}
+
+// Readers that carry a WJQ_DEC_BUF-byte digit buffer (dec_buf()) decide
+// every number; the others leave the rare hard ones undecided.
+template <class R, class = void>
+struct has_dec_buf : std::false_type {};
+template <class R>
+struct has_dec_buf<R, std::void_t<decltype(&R::dec_buf)>> : std::true_type {};
+
 struct NumVal {
   double v;
   int64_t iv;    // exact integer value when is_int
@@ -588,10 +805,18 @@ WJQ_HD bool parse_number(R &rd, uint32_t a, uint32_t b, NumVal &out) {
     return true;
   }
   uint64_t bits;
-  if (!eisel_lemire(w, q, bits)) return false;
-  if (trunc) {
+  bool ok = eisel_lemire(w, q, bits);
+  if (ok && trunc) {
     uint64_t bits2;
-    if (!eisel_lemire(w + 1, q, bits2) || bits2 != bits) return false;
+    ok = eisel_lemire(w + 1, q, bits2) && bits2 == bits;
+  }
+  if (!ok) {
+    // a reader with a digit buffer (the wave-wide phase) settles it exactly
+    if constexpr (has_dec_buf<R>::value) {
+      out.v = bits_to_f64(slow_decimal(rd, a, b, rd.dec_buf()));
+      return true;
+    }
+    return false;
   }
   double d = bits_to_f64(bits);
   out.v = neg ? -d : d;
@@ -697,6 +922,14 @@ WJQ_HD void close_hits(uint32_t F, uint32_t i, SlotV (&sl)[NS]) {
   for (int s = 0; s < NS; s++)
     if ((F >> s) & 1) sl[s].len = i - sl[s].off;
 }
+
+// Readers that hold container levels past WJQ_MAX_DEPTH (deep_cap(),
+// deep_set(level, is_array), deep_get(level)) let the walk go deeper; with
+// the others a deeper line is declined.
+template <class R, class = void>
+struct has_deep_stack : std::false_type {};
+template <class R>
+struct has_deep_stack<R, std::void_t<decltype(&R::deep_cap)>> : std::true_type {};
 
 // Validates the value starting at `i` and the whitespace after it up to the
 // end of the line; captures fused slots. Returns ST_OK / ST_INVALID /
@@ -812,9 +1045,16 @@ WJQ_HD int walk(R &rd, uint32_t i, const PP &P, SlotV (&sl)[NS], uint32_t &te, u
           state = AFTER;
           continue;
         }
-        if (d >= WJQ_MAX_DEPTH) return ST_FALLBACK;
-        if (kind == JK_ARR) arr |= 1ull << d;
-        else arr &= ~(1ull << d);
+        if (d < WJQ_MAX_DEPTH) {
+          if (kind == JK_ARR) arr |= 1ull << d;
+          else arr &= ~(1ull << d);
+        } else if constexpr (has_deep_stack<R>::value) {
+          // the wave-wide phase keeps deeper levels in LDS
+          if (d >= rd.deep_cap()) return ST_FALLBACK;
+          rd.deep_set(d, kind == JK_ARR);
+        } else {
+          return ST_FALLBACK;
+        }
         if (d < 4) {
           act = lvl_set(act, d, next_act);
           fin = lvl_set(fin, d, hit_fin);
@@ -856,7 +1096,9 @@ WJQ_HD int walk(R &rd, uint32_t i, const PP &P, SlotV (&sl)[NS], uint32_t &te, u
       i = skip_ws(rd, i);
       c = rd.byte(i);
     }
-    const bool top_arr = (arr >> (d - 1)) & 1;
+    bool top_arr;
+    if constexpr (has_deep_stack<R>::value) top_arr = d - 1 < WJQ_MAX_DEPTH ? ((arr >> (d - 1)) & 1) : rd.deep_get(d - 1);
+    else top_arr = (arr >> (d - 1)) & 1;
     if (c == ',') {
       i++;
       c = rd.byte(i);
