@@ -90,6 +90,13 @@ void warpjq_amd_device_free(uint8_t *p);
  * groups[].key_off indexes. Valid until the next submit on the slot. No
  * counterpart in the reference. */
 const uint32_t *warpjq_amd_fallback_spans(const warpjq_ctx *ctx, uint32_t slot);
+
+/* After warpjq_wait: the chunk-relative index of the first line counted as a
+ * type error (n_type_error), 0xFFFFFFFF when none, the way first_invalid
+ * reports malformed lines. A strict run stops at the lower of the two, as
+ * the reference CPU backend does (crates/warpjq-core/src/exec/cpu.rs:401-405);
+ * the reference's device ABI has no such field. */
+uint32_t warpjq_amd_first_type_error(const warpjq_ctx *ctx, uint32_t slot);
 const uint8_t *warpjq_amd_group_keys(const warpjq_ctx *ctx, uint32_t slot);
 
 /* Device time of the last submit on `slot` (hipEvent on the slot's stream):

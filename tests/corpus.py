@@ -158,3 +158,34 @@ def nginx_line(rng, i):
 def nginx(seed, lines):
     rng = random.Random(seed)
     return ("\n".join(nginx_line(rng, i) for i in range(lines)) + "\n").encode()
+
+
+def hard_numbers(rng, n):
+    """JSON numbers a 128-bit decimal-to-binary path tends to leave open:
+    exact midpoints between adjacent doubles written in full, values a hair
+    above and below them in 20-40 digits, long mantissas with large
+    exponents, subnormals."""
+    import struct
+    from decimal import Decimal, getcontext
+    getcontext().prec = 1200
+    out = []
+    while len(out) < n:
+        b = (rng.getrandbits(52)) | (rng.randrange(1, 0x7FE) << 52)
+        x = struct.unpack("<d", struct.pack("<Q", b))[0]
+        y = struct.unpack("<d", struct.pack("<Q", b + 1))[0]
+        mid = (Decimal(x) + Decimal(y)) / 2
+        k = rng.randrange(4)
+        if k == 0:
+            s = format(mid, "e")
+        elif k == 1:
+            s = format(mid + Decimal(10) ** (mid.adjusted() - 30), ".35e")
+        elif k == 2:
+            s = format(mid - Decimal(10) ** (mid.adjusted() - 30), ".35e")
+        else:
+            s = "%de%d" % (rng.randrange(10 ** 20, 10 ** 30), rng.randrange(-340, 280))
+        if rng.random() < 0.3:
+            s = "-" + s
+        out.append(s.replace("E", "e"))
+    out.append("4.9406564584124654e-324")
+    out.append("2.4703282292062327208828439643411068618252990130716238221279284125033775363510437593264991818081799618989828234772285886546332835517796989819938739800539093906315035659515570226392290858392449105184435931802849936536152500319370457678249219365623669863658480757001585769269903706311928279558551332927834338409351978015531246597263579574622766465272827220056374006485499977096599470454020828166226237857393450736339007967761930577506740176324673600968951340535537458516661134223766678604162159680461914467291840300530057530849048765391711386591646239524912623653881879636239373280423891018672348497668235089863388587925628302755995657524455507255189313690836254779186948667994968324049705821028513185451396213837722826145437693412532098591327667236328125e-324")
+    return out

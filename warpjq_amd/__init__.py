@@ -109,7 +109,8 @@ ADDITIVE_SYMBOLS = ["warpjq_amd_ext_value", "warpjq_amd_wait_ext", "warpjq_amd_s
                     "warpjq_amd_server_submit", "warpjq_amd_server_wait", "warpjq_amd_server_get_stats",
                     "warpjq_amd_ctx_reprogram", "warpjq_amd_set_graphs", "warpjq_amd_autotune",
                     "warpjq_amd_free", "warpjq_amd_compile", "warpjq_amd_program_view",
-                    "warpjq_amd_program_free", "warpjq_amd_gen_nginx", "warpjq_amd_gen_nested"]
+                    "warpjq_amd_program_free", "warpjq_amd_gen_nginx", "warpjq_amd_gen_nested",
+                    "warpjq_amd_first_type_error"]
 
 
 def build(jobs=8):
@@ -442,6 +443,14 @@ class Context:
         if rc != 0:
             raise DeviceError(f"wait: {STATUS_NAMES.get(rc, rc)}: {e.value.decode()}")
         return r
+
+    def first_type_error(self, slot):
+        """Chunk-relative index of the first type-error line of the last wait
+        on `slot` (None when there was none)."""
+        L = lib()
+        L.warpjq_amd_first_type_error.restype = ctypes.c_uint32
+        v = L.warpjq_amd_first_type_error(self.ctx, slot)
+        return None if v == 0xFFFFFFFF else v
 
     def ext(self, slot):
         x = AggExt()

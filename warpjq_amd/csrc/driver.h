@@ -24,6 +24,7 @@ struct Piece {
   const uint8_t *p;  // host bytes (always: overflow splits and row cuts need them)
   uint64_t len;
   const uint8_t *d = nullptr;  // the same bytes resident in device memory, or null
+  bool big = false;  // one line longer than the engine's chunk: run alone on a context sized for it
 };
 using Batch = std::vector<Piece>;
 
@@ -31,7 +32,9 @@ using Batch = std::vector<Piece>;
 // chains files (chunk.rs:46-57, :198-203): a chunk never crosses an input,
 // each input's BOM is dropped (chunk.rs:210-212) and its last line is
 // complete at its end. Whole small inputs share a batch; a large one is cut
-// into newline-aligned chunks (gpu/mod.rs:697-810).
+// into newline-aligned chunks (gpu/mod.rs:697-810). A line longer than the
+// chunk but within max_line comes alone, as a batch of one `big` piece
+// (gpu/mod.rs:411-458, :824-857).
 struct Batcher {
   const uint8_t *const *files;
   const uint64_t *lens;

@@ -343,9 +343,14 @@ static int drain(Run &r, uint32_t slot, const Batch &b, std::vector<Batch> &redo
   r.st.malformed += res.n_invalid;
   r.st.type_errors += res.n_type_error;
   r.st.gpu_fallback_lines += res.n_fallback;
-  if (res.n_invalid && r.oi == OI_ABORT) {
-    r.err_line = res.first_invalid == 0xFFFFFFFFu ? first_line : first_line + res.first_invalid;
-    r.err = "line is not valid JSON";
+  if ((res.n_invalid || res.n_type_error) && r.oi == OI_ABORT) {
+    // --strict stops at the first malformed or type-error line, whichever
+    // comes first (exec/cpu.rs:401-405)
+    const uint32_t fi = res.n_invalid ? res.first_invalid : 0xFFFFFFFFu;
+    const uint32_t ft = res.n_type_error ? warpjq_amd_first_type_error(r.ctx, slot) : 0xFFFFFFFFu;
+    const uint32_t f = std::min(fi, ft);
+    r.err_line = f == 0xFFFFFFFFu ? first_line : first_line + f;
+    r.err = ft < fi ? "a path indexes a value of the wrong kind" : "line is not valid JSON";
     return 2;
   }
   if (res.n_fallback) {
@@ -384,9 +389,10 @@ static int drain(Run &r, uint32_t slot, const Batch &b, std::vector<Batch> &redo
 }
 
 // Newline-aligned chunks of at most `cap` bytes holding at most `max_lines`
-// lines; a line longer than `cap` is an error (gpu/mod.rs:697-810).
+// lines (gpu/mod.rs:697-810). When the line at `pos` alone is longer than
+// `cap`, returns false with *too_long set.
 static bool next_chunk(const uint8_t *d, uint64_t n, uint64_t &pos, uint64_t cap, uint64_t max_lines,
-                       uint64_t max_line, Chunk &c, std::string &err) {
+                       uint64_t max_line, Chunk &c, std::string &err, bool *too_long = nullptr) {
   if (pos >= n) return false;
   uint64_t want = std::min<uint64_t>(cap, n - pos);
   uint64_t end;
@@ -396,6 +402,7 @@ static bool next_chunk(const uint8_t *d, uint64_t n, uint64_t &pos, uint64_t cap
     const uint8_t *nl = (const uint8_t *)memrchr(d + pos, '\n', want);
     if (!nl) {
       err = "a line is longer than the device chunk (" + std::to_string(cap) + " bytes)";
+      if (too_long) *too_long = true;
       return false;
     }
     end = (uint64_t)(nl - d) + 1;
@@ -488,8 +495,26 @@ bool Batcher::next(Batch &b, std::string &err) {
     uint64_t p = pos;
     Chunk c;
     std::string e;
-    if (!next_chunk(d, n, p, room, max_lines, max_line, c, e)) {
+    bool too_long = false;
+    if (!next_chunk(d, n, p, room, max_lines, max_line, c, e, &too_long)) {
       if (!b.empty()) break;  // no newline within the room left: next batch
+      if (too_long) {
+        // one line longer than the chunk: alone, if the limit allows it
+        // (gpu/mod.rs:411-458; a line past the limit is an error,
+        // chunk.rs:292-387)
+        const uint8_t *nl = (const uint8_t *)memchr(d + pos, '\n', n - pos);
+        const uint64_t end = nl ? (uint64_t)(nl - d) + 1 : n;
+        const uint64_t body = end - pos - (nl ? 1 : 0);
+        if (body > max_line) {
+          err = "a line exceeds the " + std::to_string(max_line) + " byte limit";
+          return false;
+        }
+        Piece pc{d + pos, end - pos, dfiles ? dfiles[fi] + pos : nullptr};
+        pc.big = true;
+        b.push_back(pc);
+        pos = end;
+        return true;
+      }
       err = e.empty() ? "a line is longer than the device chunk" : e;
       return false;
     }
@@ -641,6 +666,44 @@ int engine_run(Engine &g, Batcher &src, int format, int on_invalid, uint8_t **ou
   uint32_t slot = 0;
   int rc = 0;
   std::string cerr;
+  // a line longer than the chunk runs alone, in order, on a context sized
+  // for it (kept for the rest of the run)
+  warpjq_ctx *big = nullptr;
+  uint64_t big_cap = 0;
+  auto run_big = [&](const Batch &b) -> int {
+    const uint64_t need = b[0].len;
+    if (!big || big_cap < need) {
+      if (big) warpjq_ctx_destroy(big);
+      big = nullptr;
+      // room for the line and for rows up to 1.5x the capacity
+      uint64_t c = std::max<uint64_t>(need + (need >> 1), 1ull << 20);
+      c = std::min<uint64_t>((c + (1u << 20) - 1) & ~(uint64_t)((1u << 20) - 1), (4ull << 30) - (1ull << 20));
+      if (c < need) {
+        r.err = "a line is longer than the device can take in one chunk";
+        return 3;
+      }
+      char e[512] = {0};
+      if (warpjq_ctx_create(&g.view, c, 1, &big, e, sizeof e) != WARPJQ_OK) {
+        r.err = std::string("creating a context for a long line: ") + e;
+        big = nullptr;
+        return 3;
+      }
+      big_cap = c;
+    }
+    warpjq_ctx *keep = r.ctx;
+    r.ctx = big;
+    int rc2 = submit(r, 0, b);
+    if (!rc2) {
+      std::vector<Batch> redo;
+      rc2 = drain(r, 0, b, redo);
+      if (!rc2 && !redo.empty()) {
+        r.err = "a single line exceeds the device's per-chunk capacity";
+        rc2 = 3;
+      }
+    }
+    r.ctx = keep;
+    return rc2;
+  };
   for (;;) {
     Batch c;
     bool got;
@@ -657,6 +720,21 @@ int engine_run(Engine &g, Batcher &src, int format, int on_invalid, uint8_t **ou
       }
     }
     if (!got) break;
+    if (c.size() == 1 && c[0].big) {
+      // everything before it first, in order
+      if (have_prev) {
+        std::vector<Batch> redo;
+        if ((rc = take(slot ^ 1, prev, redo))) break;
+        have_prev = false;
+        if (!redo.empty()) {
+          redo.push_back(c);
+          pending.insert(pending.begin(), redo.begin(), redo.end());
+          continue;
+        }
+      }
+      if ((rc = run_big(c))) break;
+      continue;
+    }
     if ((rc = put(slot, c))) break;
     if (have_prev) {
       std::vector<Batch> redo;
@@ -703,6 +781,7 @@ int engine_run(Engine &g, Batcher &src, int format, int on_invalid, uint8_t **ou
       char e2[512];
       warpjq_wait(r.ctx, sl, &tmp, e2, sizeof e2);
     }
+  if (big) warpjq_ctx_destroy(big);
   stats->bytes_in = r.st.bytes_in;
   stats->lines_in = r.st.lines_in;
   stats->malformed = r.st.malformed;

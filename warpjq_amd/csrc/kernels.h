@@ -89,6 +89,7 @@ using namespace wjqd;
 #define TAB_LDS_MAX 16384u           // program tables staged in LDS per workgroup
 #define MAX_SPANS 65536u             // spans per submit
 #define KEYBLOB_BYTES (16u << 20)    // group-key bytes returned by a multi-span submit
+#define DEEP_CAP 4096u               // selected lines nested past 64 levels per submit (row outputs)
 
 // ---------------------------------------------------------------------------
 // device-side structures
@@ -107,7 +108,10 @@ struct DevCtr {
   unsigned int fb_count, n_groups;
   unsigned int rticket[NREG * RSTRIDE];  // one claim ticket per region, 128 B apart
   unsigned int blob_used;   // bytes of group keys copied out (multi-span submits)
-  unsigned int long_count, long_next;  // deferred long lines: added, taken
+  unsigned int long_count, long_next;  // lines for the wave-wide phase: added, taken
+  unsigned long long first_te_bad;  // the first type-error line: its index (row outputs) or (claim << 32 | rank)
+  unsigned int first_te;            // ... as a line index after the pass
+  unsigned int deep_count;          // selected lines past 64 levels whose captures k_emit reads back
 };
 
 // Per-wave ungrouped aggregate partial.
@@ -119,7 +123,7 @@ struct AggPart {
 };
 
 struct GroupTab {
-  unsigned long long *keys;  // packed off:40 | len:19 | esc:1 | kind:4
+  unsigned long long *keys;  // packed off:32 | len:27 | esc:1 | kind:4
   unsigned int *hash;
   unsigned long long *count, *numeric, *isum_lo, *isum_hi, *mnb, *mxb;
   double *fsum, *fcomp;
@@ -161,6 +165,11 @@ struct KArgs {
   // {q start, rec, tile + 1} each, the last word written last (0 = not yet)
   uint32_t *long_q;
   uint32_t long_cap;
+  // row outputs: captures of selected lines nested past 64 levels, which the
+  // per-lane re-walk in k_emit cannot redo
+  uint32_t *deep_rec, *deep_end;
+  SlotV *deep_sl;  // 16 per entry
+  uint32_t deep_cap;
 };
 
 // ---------------------------------------------------------------------------
@@ -245,15 +254,39 @@ struct LdsReader {
 // wait per window), instead of one dependent HBM load per 16 bytes. Every
 // lane must read the same positions in step: the long-line phase walks one
 // line with all 64 lanes. Positions at or past n read as '\n'.
+//
+// Past the window the arena holds a digit buffer (numbers the 128-bit path
+// leaves open are converted exactly there, every lane computing the same
+// digits) and a bit stack for container levels past the 64 kept in
+// registers, so this phase decides lines the per-lane walk declines.
 struct WinReader {
   const uint8_t *d;
   uint32_t n;
   LDS_AS uint8_t *w;
   uint32_t wblk;  // window size in 16-byte blocks (a multiple of 64)
   uint32_t wb;    // first block held
+  LDS_AS uint8_t *ext;  // WJQ_DEC_BUF digit bytes, then the deep stack
+  uint32_t dcap;        // container levels the walk may open
+  bool deep_used;       // the line went past the levels kept in registers
 
-  __device__ WinReader(const uint8_t *d_, uint32_t n_, LDS_AS uint8_t *w_, uint32_t wbytes)
-      : d(d_), n(n_), w(w_), wblk(wbytes >> 4), wb(0x80000000u) {}
+  __device__ WinReader(const uint8_t *d_, uint32_t n_, LDS_AS uint8_t *w_, uint32_t wbytes, uint32_t ext_bytes)
+      : d(d_), n(n_), w(w_), wblk(wbytes >> 4), wb(0x80000000u), ext(w_ + wbytes),
+        dcap(WJQ_MAX_DEPTH + (ext_bytes - WJQ_DEC_BUF) * 8u), deep_used(false) {}
+
+  __device__ __forceinline__ LDS_AS uint8_t *dec_buf() { return ext; }
+  __device__ __forceinline__ uint32_t deep_cap() const { return dcap; }
+  __device__ __forceinline__ void deep_set(uint32_t lvl, bool is_arr) {
+    const uint32_t k = lvl - WJQ_MAX_DEPTH;
+    LDS_AS uint8_t *p = ext + WJQ_DEC_BUF + (k >> 3);
+    const uint32_t m = 1u << (k & 7u);
+    const uint32_t v = *p;
+    *p = (uint8_t)(is_arr ? (v | m) : (v & ~m));
+    deep_used = true;
+  }
+  __device__ __forceinline__ bool deep_get(uint32_t lvl) const {
+    const uint32_t k = lvl - WJQ_MAX_DEPTH;
+    return (ext[WJQ_DEC_BUF + (k >> 3)] >> (k & 7u)) & 1u;
+  }
 
   // The window from block b on (every lane takes part).
   __device__ __forceinline__ void fill(uint32_t b) {
@@ -458,14 +491,17 @@ __device__ __forceinline__ uint4 end_patch(uint4 v, uint32_t p, uint32_t n) {
 // group table
 // ---------------------------------------------------------------------------
 
+// off:32 | len:27 | esc:1 | kind:4. EMPTY_KEY (all ones) would need off +
+// len >= 2^32, which no key reaches.
+#define GKEY_LEN_MAX 0x7FFFFFFu
 __device__ __forceinline__ unsigned long long pack_key(const GKey &k) {
-  return ((unsigned long long)k.off << 24) | ((unsigned long long)(k.len & 0x7FFFF) << 5) |
+  return ((unsigned long long)k.off << 32) | ((unsigned long long)(k.len & GKEY_LEN_MAX) << 5) |
          ((unsigned long long)(k.esc & 1) << 4) | (k.kind & 0xF);
 }
 __device__ __forceinline__ GKey unpack_key(unsigned long long p) {
   GKey k;
-  k.off = (uint32_t)(p >> 24);
-  k.len = (uint32_t)((p >> 5) & 0x7FFFF);
+  k.off = (uint32_t)(p >> 32);
+  k.len = (uint32_t)((p >> 5) & GKEY_LEN_MAX);
   k.esc = (uint32_t)((p >> 4) & 1);
   k.kind = (uint32_t)(p & 0xF);
   return k;
@@ -677,6 +713,7 @@ __device__ __forceinline__ void group_line(R &rd, const Span *spans, uint32_t n_
 // Per-lane tallies, flushed once at the end of the kernel.
 struct LaneAcc {
   uint32_t blank, inv, te, fb, sel;
+  uint64_t te_first;  // lowest type-error record (line index, or claim << 32 | rank)
   uint64_t cnt, num, ilo, ihi;
   double fs, fc, mn, mx;
   bool saw_nn;
@@ -685,6 +722,7 @@ struct LaneAcc {
 
 __device__ __forceinline__ void acc_zero(LaneAcc &L) {
   L.blank = L.inv = L.te = L.fb = L.sel = 0;
+  L.te_first = ~0ull;
   L.cnt = L.num = L.ilo = L.ihi = 0;
   L.fs = L.fc = 0.0;
   L.mn = INFINITY;
@@ -698,6 +736,7 @@ __device__ __forceinline__ void acc_merge(LaneAcc &L, const LaneAcc &d) {
   L.blank += d.blank;
   L.inv += d.inv;
   L.te += d.te;
+  if (d.te_first < L.te_first) L.te_first = d.te_first;
   L.fb += d.fb;
   L.sel += d.sel;
   L.cnt += d.cnt;
@@ -722,9 +761,14 @@ __device__ __forceinline__ void acc_merge(LaneAcc &L, const LaneAcc &d) {
 // chunk (row outputs) or its rank in claim `tile` (aggregates).
 // `act` false: evaluate and tally into L only (the long-line phase runs one
 // line on every lane; lane 0 alone touches shared state).
+// Returns true when a per-lane reader could not decide the line (nesting past
+// 64 levels, a number the 128-bit path leaves open, a group key longer than
+// a window): nothing is tallied and the caller hands the line to the
+// wave-wide phase, whose reader decides it.
 template <int NS, bool GRP, bool LB, class R>
-__device__ __forceinline__ void process_line(R &rd, const KArgs &a, const LProg &P, uint32_t s, uint32_t rec,
+__device__ __forceinline__ bool process_line(R &rd, const KArgs &a, const LProg &P, uint32_t s, uint32_t rec,
                                              uint32_t tile, LaneAcc &L, LGroup *lgp, bool act = true) {
+  constexpr bool WIDE = wide_reader<R>::value;
   SlotV sl[NS];
   LineResult r = eval_line<NS>(rd, s, P, sl);
   if (r.status == ST_OK && r.selected && P.output_kind == WARPJQ_OUT_AGG) {
@@ -751,17 +795,21 @@ __device__ __forceinline__ void process_line(R &rd, const KArgs &a, const LProg 
     uint32_t gh = 0;
     if (!undecided && GRP) {
       gk = group_key(rd, pick<NS>(sl, P.group_slot));
-      if (gk.len > 0x7FFFF) undecided = true;
+      if (gk.len > GKEY_LEN_MAX) undecided = true;
       else gh = group_hash(rd, gk);
-      // the long-line phase: lane 0 alone re-reads the key in group_line
-      if constexpr (wide_reader<R>::value) {
-        if (!undecided && !rd.ensure(gk.off, gk.off + gk.len)) undecided = true;
-      }
     }
     if (undecided) {
       r.status = ST_FALLBACK;
     } else if (GRP) {
-      if (act) group_line(rd, a.spans, a.n_spans, a.gt, *lgp, gk, gh, has_num, is_int, iv, v, L.govf);
+      if constexpr (WIDE) {
+        // lane 0 alone inserts: it reads the key from HBM, not the window
+        if (act) {
+          DevReader kr = span_reader(a.spans, a.n_spans, gk.off);
+          group_line(kr, a.spans, a.n_spans, a.gt, *lgp, gk, gh, has_num, is_int, iv, v, L.govf);
+        }
+      } else {
+        if (act) group_line(rd, a.spans, a.n_spans, a.gt, *lgp, gk, gh, has_num, is_int, iv, v, L.govf);
+      }
     } else {
       L.cnt++;
       if (has_num) {
@@ -782,12 +830,28 @@ __device__ __forceinline__ void process_line(R &rd, const KArgs &a, const LProg 
       }
     }
   }
+  if constexpr (!WIDE) {
+    if (r.status == ST_FALLBACK) return true;
+  }
   if (LB) {
     uint32_t rl = 0;
     if (r.status == ST_OK && r.selected) {
       uint64_t n = row<NS, R, DevSink>(rd, P, sl, s, r.end, nullptr);
       rl = (uint32_t)n;
       L.sel++;
+      if constexpr (WIDE) {
+        if (rd.deep_used && act) {
+          const uint32_t k = atomicAdd(&a.ctr->deep_count, 1u);
+          if (k < a.deep_cap) {
+            a.deep_rec[k] = rec;
+            a.deep_end[k] = r.end;
+#pragma unroll
+            for (int q = 0; q < NS; q++) a.deep_sl[16u * k + q] = sl[q];
+          } else {
+            L.ovf = 1;
+          }
+        }
+      }
     }
     if (rec < a.max_lines) {
       if (act) {
@@ -807,8 +871,15 @@ __device__ __forceinline__ void process_line(R &rd, const KArgs &a, const LProg 
         else atomicMin(&a.ctr->first_bad, ((unsigned long long)tile << 32) | rec);
       }
       break;
-    case ST_TYPE_ERROR: L.te++; break;
+    case ST_TYPE_ERROR: {
+      L.te++;
+      const uint64_t tr = LB ? (uint64_t)rec : (((uint64_t)tile << 32) | rec);
+      if (tr < L.te_first) L.te_first = tr;
+      break;
+    }
     case ST_FALLBACK: {
+      // only the wave-wide phase gets here: nesting past its LDS stack, or a
+      // group key past 128 MiB
       L.fb++;
       uint32_t e = s;
       while (rd.byte(e) != WJQ_END) e++;
@@ -830,6 +901,7 @@ __device__ __forceinline__ void process_line(R &rd, const KArgs &a, const LProg 
     }
     default: break;
   }
+  return false;
 }
 
 // ---------------------------------------------------------------------------
@@ -1218,11 +1290,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       const uint32_t rank = qv >> 19;
       const uint32_t rec = LB ? Q.cr_base[slot] + rank : rank;
       const uint32_t tile = Q.cr_t[slot];
-      if (!(qv & QE_UNSTAGED)) {
+      bool wide = (qv & QE_UNSTAGED) != 0;  // too long to stage
+      if (!wide) {
         LdsReader rd(ar, off - s);
-        process_line<NS, GRP, LB>(rd, a, P, s, rec, tile, L, lgp);
-      } else {
-        // too long to stage: walked by the whole wave after the claims
+        wide = process_line<NS, GRP, LB>(rd, a, P, s, rec, tile, L, lgp);  // or declined per lane
+      }
+      if (wide) {
+        // walked by the whole wave after the claims; a full queue makes the
+        // host split the chunk
         const uint32_t k = atomicAdd(&a.ctr->long_count, 1u);
         if (k < a.long_cap) {
           a.long_q[3 * k] = s;
@@ -1230,8 +1305,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
           __threadfence();
           __hip_atomic_store(&a.long_q[3 * k + 2], tile + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-          DevReader rd = span_reader(a.spans, a.n_spans, s);
-          process_line<NS, GRP, LB>(rd, a, P, s, rec, tile, L, lgp);
+          L.ovf = 1;
         }
       }
     }
@@ -1245,11 +1319,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     __syncthreads();
   }
 
-  // ---- long lines: one at a time, every lane walking it in step through
-  // the arena (free now) as a window. Each line is taken by some wave: the
-  // wave that added it looks here after adding it.
+  // ---- long lines (and lines the per-lane walk declined): one at a time,
+  // every lane walking it in step through the arena (free now) as a window,
+  // the last 4 KiB (less in small arenas) kept for the digit buffer and the
+  // deep container stack. Each line is taken by some wave: the wave that
+  // added it looks here after adding it.
   {
-    const uint32_t wbytes = a.arena & ~1023u;
+    const uint32_t wbytes = a.arena >= 5120u ? ((a.arena - 4096u) & ~1023u) : 1024u;
     for (;;) {
       uint32_t k = ~0u;
       if (lane == 0) {
@@ -1278,7 +1354,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       lrec = __shfl(lrec, 0, 64);
       lt = __shfl(lt, 0, 64);
       const Span sp = a.spans[span_of(a.spans, a.n_spans, ls)];
-      WinReader rd(sp.base - sp.vbase, sp.vend, ar, wbytes);
+      WinReader rd(sp.base - sp.vbase, sp.vend, ar, wbytes, a.arena - wbytes);
       LaneAcc D;
       acc_zero(D);
       process_line<NS, GRP, LB>(rd, a, P, ls, lrec, lt - 1u, D, lgp, lane == 0);
@@ -1291,6 +1367,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   unsigned long long sb = wave_sum((unsigned long long)L.blank), si = wave_sum((unsigned long long)L.inv),
                      st = wave_sum((unsigned long long)L.te), sf = wave_sum((unsigned long long)L.fb),
                      ss = wave_sum((unsigned long long)L.sel);
+  uint64_t tf = L.te_first;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t x = __shfl_xor(tf, o, 64);
+    if (x < tf) tf = x;
+  }
+  if (lane == 0 && tf != ~0ull) atomicMin(&a.ctr->first_te_bad, (unsigned long long)tf);
   uint64_t nn = __ballot(L.saw_nn);
   uint64_t ov = __ballot(L.ovf != 0);
   uint64_t gov = __ballot(L.govf != 0);
@@ -1375,8 +1458,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 // One lane per selected row: re-evaluate and write its bytes.
 template <int NS>
 __global__ __launch_bounds__(256) void k_emit(const Span *spans, uint32_t n_spans, DProg P0, const uint8_t *tab,
-                                               uint32_t tab_bytes, const DevCtr *ctr, const uint32_t *sel_idx,
-                                               const uint64_t *row_off, const uint32_t *sel_start, uint8_t *out) {
+                                               uint32_t tab_bytes, DevCtr *ctr, const uint32_t *sel_idx,
+                                               const uint64_t *row_off, const uint32_t *sel_start, uint8_t *out,
+                                               const uint32_t *deep_rec, const uint32_t *deep_end,
+                                               const SlotV *deep_sl, uint32_t deep_cap) {
   const LProg P = stage_prog(P0, tab, tab_bytes, (LDS_AS uint8_t *)wjq_dyn_lds);
   if (ctr->overflow) return;
   uint64_t nsel = ctr->n_sel;
@@ -1403,8 +1488,23 @@ __global__ __launch_bounds__(256) void k_emit(const Span *spans, uint32_t n_span
     DevReader rd = span_reader(spans, n_spans, s);
     SlotV sl[NS];
     LineResult lr = eval_line<NS>(rd, s, P, sl);
+    if (lr.status != ST_OK) {
+      // A selected line is re-walked here by one lane. Nested past the 64
+      // levels a lane keeps, the walk stops early: the wave-wide phase stored
+      // its captures. A number this lane cannot settle stops only the filter
+      // (end is set), and the captures are whole.
+      bool found = false;
+      const uint32_t nd = min(ctr->deep_count, deep_cap);
+      for (uint32_t k = 0; !found && k < nd; k++)
+        if (deep_rec[k] == idx) {
+#pragma unroll
+          for (int q = 0; q < NS; q++) sl[q] = deep_sl[16u * k + q];
+          lr.end = deep_end[k];
+          found = true;
+        }
+      if (!found && !(lr.status == ST_FALLBACK && lr.end != 0)) atomicOr(&ctr->overflow, 1u);
+    }
     DevSink w{out, row_off[r]};
-    (void)lr;
     row<NS, DevReader, DevSink>(rd, P, sl, s, lr.end, &w);
   }
 }
@@ -1417,8 +1517,8 @@ __global__ __launch_bounds__(256) void k_emit(const Span *spans, uint32_t n_span
   void launch_main_##NS(bool grouped, bool lb, uint32_t grid, hipStream_t st, const KArgs &a);          \
   int occupancy_main_##NS(bool grouped, bool lb, uint32_t lds);                                         \
   void launch_emit_##NS(uint32_t grid, hipStream_t st, const Span *spans, uint32_t n_spans, const DProg &P, \
-                        const uint8_t *tab, uint32_t tab_bytes, const DevCtr *ctr, const uint32_t *sel_idx, \
-                        const uint64_t *row_off, const uint32_t *sel_start, uint8_t *out);
+                        const uint8_t *tab, uint32_t tab_bytes, DevCtr *ctr, const uint32_t *sel_idx,       \
+                        const uint64_t *row_off, const uint32_t *sel_start, uint8_t *out, const KArgs &a);
 WJQ_DECLARE_LAUNCHERS(1)
 WJQ_DECLARE_LAUNCHERS(2)
 WJQ_DECLARE_LAUNCHERS(4)
@@ -1441,8 +1541,9 @@ WJQ_DECLARE_LAUNCHERS(16)
     return e == hipSuccess ? b : 0;                                                                       \
   }                                                                                                       \
   void launch_emit_##NS(uint32_t grid, hipStream_t st, const Span *spans, uint32_t n_spans, const DProg &P, \
-                        const uint8_t *tab, uint32_t tab_bytes, const DevCtr *ctr, const uint32_t *sel_idx, \
-                        const uint64_t *row_off, const uint32_t *sel_start, uint8_t *out) {              \
+                        const uint8_t *tab, uint32_t tab_bytes, DevCtr *ctr, const uint32_t *sel_idx,       \
+                        const uint64_t *row_off, const uint32_t *sel_start, uint8_t *out, const KArgs &a) { \
     hipLaunchKernelGGL((k_emit<NS>), dim3(grid), dim3(256), (tab_bytes + 15u) & ~15u, st, spans, n_spans, P, tab, \
-                       tab_bytes, ctr, sel_idx, row_off, sel_start, out);                                \
+                       tab_bytes, ctr, sel_idx, row_off, sel_start, out, a.deep_rec, a.deep_end, a.deep_sl,     \
+                       a.deep_cap);                                                                        \
   }

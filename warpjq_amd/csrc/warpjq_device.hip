@@ -20,6 +20,9 @@ __global__ void k_reset(DevCtr *ctr, unsigned long long *tile_state, uint32_t nt
 
     ctr->blob_used = 0;
     ctr->long_count = ctr->long_next = 0;
+    ctr->first_te_bad = ~0ull;
+    ctr->first_te = 0xFFFFFFFFu;
+    ctr->deep_count = 0;
   }
   if (i < NREG) ctr->rticket[i * RSTRIDE] = 0;
   for (uint32_t t = i; t < ntiles; t += gridDim.x * blockDim.x) tile_state[t] = 0;
@@ -439,6 +442,10 @@ __global__ void k_res_records(DevCtr *ctr, const unsigned long long *prefix, uin
     unsigned long long b = ctr->first_bad;
     ctr->first_invalid = (uint32_t)(prefix[b >> 32] + (b & 0xFFFFFFFFull));
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && ctr->first_te_bad != ~0ull) {
+    unsigned long long b = ctr->first_te_bad;
+    ctr->first_te = (uint32_t)(prefix[b >> 32] + (b & 0xFFFFFFFFull));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -489,7 +496,9 @@ struct Slot {
   // the submit's spans (device copy, pinned staging for it, host view)
   Span *d_spans = nullptr, *h_spans = nullptr;
   warpjq_amd_span *h_pieces = nullptr;
-  uint32_t *d_long = nullptr;  // deferred long lines, 3 words each, zero between launches
+  uint32_t *d_long = nullptr;  // lines for the wave-wide phase, 3 words each, zero between launches
+  uint32_t *d_deep_rec = nullptr, *d_deep_end = nullptr;  // row outputs: captures of lines past 64 levels
+  SlotV *d_deep_sl = nullptr;
   unsigned long long *d_tsum = nullptr;  // block totals of the line-count scan
   std::vector<GraphEntry> graphs;  // captured chains of resident submits
   uint64_t graph_tick = 0;
@@ -498,6 +507,7 @@ struct Slot {
   uint8_t *d_keyblob = nullptr, *h_keyblob = nullptr;
 
   bool submitted = false;
+  uint32_t first_te = 0xFFFFFFFFu;  // lowest type-error line index of the last wait
 };
 
 }  // namespace
@@ -592,6 +602,9 @@ static void free_slot(Slot &s) {
   (void)hipFree(s.d_fb_tile);
   (void)hipFree(s.d_spans);
   (void)hipFree(s.d_long);
+  (void)hipFree(s.d_deep_rec);
+  (void)hipFree(s.d_deep_end);
+  (void)hipFree(s.d_deep_sl);
   (void)hipFree(s.d_tsum);
   for (GraphEntry &g : s.graphs) (void)hipGraphExecDestroy(g.exec);
   s.graphs.clear();
@@ -667,6 +680,9 @@ static warpjq_status alloc_slot(warpjq_ctx *ctx, Slot &s, char *err, size_t err_
     HIP_TRY(hipGetLastError(), "k_group_init");
   }
   if (ctx->rows) {
+    HIP_TRY(dalloc(&s.d_deep_rec, DEEP_CAP), "hipMalloc(deep)");
+    HIP_TRY(dalloc(&s.d_deep_end, DEEP_CAP), "hipMalloc(deep)");
+    HIP_TRY(dalloc(&s.d_deep_sl, (size_t)DEEP_CAP * 16), "hipMalloc(deep)");
     const uint64_t ml = ctx->max_lines;
     const uint64_t nb = (ml + ROWSCAN_BLOCK - 1) / ROWSCAN_BLOCK;
     HIP_TRY(dalloc(&s.d_row_len, ml), "hipMalloc(row_len)");
@@ -921,6 +937,10 @@ static warpjq_status enqueue_chain(warpjq_ctx *ctx, Slot &s, uint32_t ns, uint32
     a.fb_tile = s.d_fb_tile;
     a.fb_cap = ctx->fb_cap;
     a.long_q = s.d_long;
+    a.deep_rec = s.d_deep_rec;
+    a.deep_end = s.d_deep_end;
+    a.deep_sl = s.d_deep_sl;
+    a.deep_cap = ctx->rows ? DEEP_CAP : 0;
     a.long_cap = ctx->max_tiles;
     uint32_t grid = std::min<uint32_t>(ctx->grid, ntiles);
     if (ctx->rows) {  // each claim's first line index, so claims can go in any order
@@ -961,11 +981,11 @@ static warpjq_status enqueue_chain(warpjq_ctx *ctx, Slot &s, uint32_t ns, uint32
       uint32_t eg = std::min<uint32_t>(ctx->grid, (uint32_t)((lines_bound + 255) / 256));
       if (eg == 0) eg = 1;
       switch (ctx->host.ns) {
-        case 1: launch_emit_1(eg, st, s.d_spans, ns, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
-        case 2: launch_emit_2(eg, st, s.d_spans, ns, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
-        case 4: launch_emit_4(eg, st, s.d_spans, ns, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
-        case 8: launch_emit_8(eg, st, s.d_spans, ns, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
-        default: launch_emit_16(eg, st, s.d_spans, ns, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out); break;
+        case 1: launch_emit_1(eg, st, s.d_spans, ns, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out, a); break;
+        case 2: launch_emit_2(eg, st, s.d_spans, ns, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out, a); break;
+        case 4: launch_emit_4(eg, st, s.d_spans, ns, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out, a); break;
+        case 8: launch_emit_8(eg, st, s.d_spans, ns, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out, a); break;
+        default: launch_emit_16(eg, st, s.d_spans, ns, ctx->dprog, (const uint8_t *)ctx->d_tables, ctx->tab_bytes, s.d_ctr, s.d_sel_idx, s.d_row_off, s.d_sel_start, s.d_out, a); break;
       }
     }
     HIP_TRY(hipGetLastError(), "kernel chain");
@@ -1154,6 +1174,11 @@ const uint32_t *warpjq_amd_fallback_spans(const warpjq_ctx *ctx, uint32_t slot) 
   return ctx->slots[slot].h_fb_span;
 }
 
+uint32_t warpjq_amd_first_type_error(const warpjq_ctx *ctx, uint32_t slot) {
+  if (!ctx || slot >= ctx->n_slots) return 0xFFFFFFFFu;
+  return ctx->slots[slot].first_te;
+}
+
 const uint8_t *warpjq_amd_group_keys(const warpjq_ctx *ctx, uint32_t slot) {
   if (!ctx || slot >= ctx->n_slots) return nullptr;
   return ctx->slots[slot].h_keyblob;
@@ -1166,9 +1191,11 @@ warpjq_status warpjq_wait(warpjq_ctx *ctx, uint32_t slot, warpjq_chunk_result *o
   HIP_TRY(hipGetLastError(), "kernel");
   memset(out, 0, sizeof *out);
   out->first_invalid = 0xFFFFFFFFu;
+  s.first_te = 0xFFFFFFFFu;
   if (!s.submitted || s.n_bytes == 0) return WARPJQ_OK;
   DevCtr c = *s.h_ctr;
-  if (!ctx->rows && !c.overflow && (c.first_bad != ~0ull || c.fb_count) && c.fb_count <= ctx->fb_cap) {
+  if (!ctx->rows && !c.overflow && (c.first_bad != ~0ull || c.first_te_bad != ~0ull || c.fb_count) &&
+      c.fb_count <= ctx->fb_cap) {
     // aggregate queries recorded (claim, rank): turn them into line indices
     hipLaunchKernelGGL(k_res_prefix, dim3(1), dim3(1024), 0, s.stream, s.d_tile_state, s.ntiles);
     hipLaunchKernelGGL(k_res_records, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (c.fb_count + 255) / 256))),
@@ -1184,6 +1211,7 @@ warpjq_status warpjq_wait(warpjq_ctx *ctx, uint32_t slot, warpjq_chunk_result *o
   out->n_type_error = c.n_type_error;
   out->n_fallback = c.n_fallback;
   out->first_invalid = c.first_invalid;
+  s.first_te = c.first_te_bad == ~0ull ? 0xFFFFFFFFu : ctx->rows ? (uint32_t)c.first_te_bad : c.first_te;
   out->chunk_overflow = c.overflow || c.fb_count > ctx->fb_cap;
   out->agg.saw_non_numeric = c.saw_non_numeric;
   if (out->chunk_overflow) {
