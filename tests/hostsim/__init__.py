@@ -37,6 +37,8 @@ def lib():
         L.hs_slow_decimal.restype = ctypes.c_double
         L.hs_validate_deep.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32]
         L.hs_validate_deep.restype = ctypes.c_int
+        L.hs_flat_vs_walk.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32]
+        L.hs_flat_vs_walk.restype = ctypes.c_int
         L.hs_validate.argtypes = [ctypes.c_char_p, ctypes.c_uint64]
         L.hs_validate.restype = ctypes.c_int
         _lib = L
@@ -53,6 +55,9 @@ class Evaluator:
     def __del__(self):
         if getattr(self, "h", None):
             lib().hs_free(self.h)
+
+    def flat_vs_walk(self, line, lead=0):
+        return lib().hs_flat_vs_walk(self.h, line, len(line), lead)
 
     def eval(self, line):
         o = HsLine()

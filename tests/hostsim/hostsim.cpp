@@ -219,6 +219,49 @@ int hs_validate_deep(const uint8_t *line, uint64_t n, uint32_t cap) {
   return walk<1>(rd, i, P, sl, te, end);
 }
 
+// The flat fast path against the byte walk on one line: 0 the fast path
+// declined it, 1 it accepted it and the walk agrees (valid, same captures,
+// type errors and end), 2 it accepted a line the walk decides otherwise.
+}  // extern "C"
+
+template <int NS>
+static int flat_vs_walk_t(Handle *h, HostReader &rd, uint32_t i) {
+  const DProg &P = h->dp.d;
+  SlotV a[NS], b[NS];
+  for (int q = 0; q < NS; q++) a[q] = b[q] = SlotV{0, 0, JK_MISSING};
+  uint32_t ta = 0, tb = 0, ea = 0, eb = 0;
+  if (!flat_walk<NS>(rd, i, P, a, ta, ea)) return 0;
+  int st = walk<NS>(rd, i, P, b, tb, eb);
+  if (st != ST_OK || ta != tb || ea != eb) return 2;
+  for (int q = 0; q < NS; q++) {
+    if (!((P.fused_mask >> q) & 1)) continue;
+    if (a[q].kind != b[q].kind) return 2;
+    if (a[q].kind != JK_MISSING && (a[q].off != b[q].off || a[q].len != b[q].len)) return 2;
+  }
+  return 1;
+}
+
+extern "C" {
+
+int hs_flat_vs_walk(void *hp, const uint8_t *line, uint64_t n, uint32_t lead) {
+  Handle *h = (Handle *)hp;
+  // `lead` bytes of junk before the line (the staged neighbour's tail)
+  std::vector<uint8_t> buf(lead, 'x');
+  buf.insert(buf.end(), line, line + n);
+  buf.push_back('\n');
+  for (int k = 0; k < 80; k++) buf.push_back(k & 1 ? '"' : ',');  // bytes past the end
+  HostReader rd{buf.data(), (uint32_t)buf.size()};
+  uint32_t i = lead;
+  if (rd.byte(i) != '{') return 0;
+  switch (h->dp.ns) {
+    case 1: return flat_vs_walk_t<1>(h, rd, i);
+    case 2: return flat_vs_walk_t<2>(h, rd, i);
+    case 4: return flat_vs_walk_t<4>(h, rd, i);
+    case 8: return flat_vs_walk_t<8>(h, rd, i);
+    default: return flat_vs_walk_t<16>(h, rd, i);
+  }
+}
+
 // The device's string scanner + structural validation on one line.
 int hs_validate(const uint8_t *line, uint64_t n) {
   std::vector<uint8_t> buf(line, line + n);

@@ -277,3 +277,57 @@ def test_deep_lines_walked_with_a_deep_stack():
         line = b'{"k":' + b"[" * n + b'{"x":1}' + b"]" * n + b"}"
         out, st = oracle.run("count", line + b"\n", threads=1)
         assert hostsim.validate_deep(line) == (0 if st["malformed"] == 0 else 1)
+
+
+def _flat_line(rng):
+    keys = ["ts", "host", "status", "bytes", "path", "a", "", "k" * 40, "é", "status"]
+    parts = []
+    for _ in range(rng.randrange(0, 9)):
+        k = rng.choice(keys)
+        v = rng.choice(['"x"', '""', '"GET /a,b:c{}[]"', str(rng.randrange(0, 10 ** rng.randrange(1, 20))), "0",
+                        '"%s"' % ("u" * rng.randrange(0, 200)), "12", '"日本"'])
+        parts.append('"%s":%s' % (k, v))
+    return "{" + ",".join(parts) + "}"
+
+
+_MUT = ['"', ",", ":", "{", "}", "[", "]", " ", "\t", "\\", "0", "1", "-", ".", "e", "n", "\r", "\x01", "\x7f", "\xff"]
+
+
+def _mutate(rng, s):
+    b = bytearray(s.encode())
+    for _ in range(rng.randrange(1, 3)):
+        k = rng.randrange(3)
+        p = rng.randrange(len(b) + 1)
+        c = rng.choice(_MUT).encode("latin-1")
+        if k == 0 and p < len(b):
+            b[p:p + 1] = c
+        elif k == 1:
+            b[p:p] = c
+        elif p < len(b):
+            del b[p]
+    return bytes(b).replace(b"\n", b"")
+
+
+def test_flat_fast_path_never_disagrees_with_the_walk():
+    # The bit-parallel fast path may only accept lines the byte walk finds
+    # valid, with the same captures, type errors and end; a fuzz of flat
+    # lines and one-or-two-byte mutations of them, at every offset within a
+    # 16-byte block and with junk after the line. It must also accept the
+    # common shape (so the kernels use it).
+    rng = random.Random(99)
+    qs = ["count", "select(.status == 500) | count", "{h: .host, s: .status, b: .bytes}", ".a.b", ".[0]",
+          "select(.path.x == 1) | .k", 'select(.host == "x") | .ts', "group_by(.status) | sum(.bytes)"]
+    evs = [hostsim.Evaluator(q) for q in qs]
+    accepted = 0
+    for it in range(12000):
+        line = _flat_line(rng).encode()
+        if it % 3:
+            line = _mutate(rng, line.decode())
+        ev = evs[it % len(evs)]
+        for lead in (0, 5, 15):
+            r = ev.flat_vs_walk(line, lead)
+            assert r != 2, (qs[it % len(evs)], line, lead)
+            accepted += r == 1
+    assert accepted > 12000
+    nginx = corpus.nginx(5, 300).split(b"\n")[:-1]
+    assert all(evs[1].flat_vs_walk(ln) == 1 for ln in nginx)

@@ -1349,6 +1349,276 @@ WJQ_HD int eval_cond(R &rd, const PP &P, const SlotV (&sl)[NS]) {
 }
 
 // ---------------------------------------------------------------------------
+// fast path: one minified flat object (bit-parallel)
+//
+// Most NDJSON lines are one object whose members are strings and unsigned
+// integers, written without whitespace. For such a line the grammar reduces
+// to relations between neighbouring bytes, checked 64 bytes at a time on
+// bit masks instead of by a byte walk:
+//   * each byte is classified by two 16-entry nibble tables (v_perm_b32 on
+//     the device) into , : " digit '0' backslash control;
+//   * the quote mask's prefix XOR gives the bytes inside strings, so the
+//     other masks are taken outside strings only;
+//   * every relation is a shift by one byte: after '{' comes '"' or '}',
+//     after ',' a key, after ':' a string or a digit run, after a closing
+//     quote ':' ',' or '}', after a digit a digit ',' or '}', and so on;
+//   * keys and values alternate: adding the key starts to the string mask
+//     carries each key's run up to its closing quote ((S + KS) & ~S), and
+//     those closing quotes must be exactly the bytes before the colons.
+// Any byte outside that shape (whitespace, escapes, signs, fractions,
+// literals, nesting, control bytes, a line longer than FLAT_WORDS x 64
+// bytes) makes it return false and the caller walks the line byte by byte,
+// which decides everything; when it returns true the line is valid and the
+// captures of the fused paths are those walk() would take. Each lane runs it
+// on its own line, so there is no carry between lanes.
+// ---------------------------------------------------------------------------
+
+#define FC_COMMA 0x01u
+#define FC_COLON 0x02u
+#define FC_QUOTE 0x04u
+#define FC_DIGIT 0x08u
+#define FC_ZERO 0x10u
+#define FC_BSL 0x20u
+#define FC_CTRL 0x40u
+#define FLAT_WORDS 5
+
+// Class bits of the four bytes of x (byte k in bits 8k..8k+7): the AND of a
+// low-nibble and a high-nibble table entry. Bytes >= 0x80 classify as 0.
+WJQ_HD uint32_t fclass4(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // low nibble: entries 0-7 and 8-15 looked up with its low three bits,
+  // chosen per byte by bit 3 (selector 12 gives 0x00, 13 gives 0xFF)
+  const uint32_t lo = x & 0x0F0F0F0Fu;
+  const uint32_t l7 = lo & 0x07070707u;
+  const uint32_t a = __builtin_amdgcn_perm(0x48484848u, 0x484C4858u, l7);
+  const uint32_t b = __builtin_amdgcn_perm(0x40404061u, 0x40424848u, l7);
+  const uint32_t m = __builtin_amdgcn_perm(0u, 0u, ((lo >> 3) & 0x01010101u) | 0x0C0C0C0Cu);
+  const uint32_t lc = (b & m) | (a & ~m);
+  // high nibble: 12-15 folded to 8-11, whose selectors give the sign of
+  // table bytes 1, 3, 5, 7 (all clear): 0 for every byte >= 0x80
+  const uint32_t hi = (x >> 4) & 0x0F0F0F0Fu;
+  const uint32_t hs = hi & ~((hi >> 1) & 0x04040404u);
+  const uint32_t hc = __builtin_amdgcn_perm(0x00002000u, 0x1A054040u, hs);
+  return lc & hc;
+#else
+  static const uint8_t LO[16] = {0x58, 0x48, 0x4C, 0x48, 0x48, 0x48, 0x48, 0x48,
+                                 0x48, 0x48, 0x42, 0x40, 0x61, 0x40, 0x40, 0x40};
+  static const uint8_t HI[16] = {0x40, 0x40, 0x05, 0x1A, 0x00, 0x20, 0x00, 0x00,
+                                 0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t r = 0;
+  for (int k = 0; k < 4; k++) {
+    const uint32_t c = (x >> (8 * k)) & 0xFFu;
+    r |= (uint32_t)(LO[c & 15] & HI[c >> 4]) << (8 * k);
+  }
+  return r;
+#endif
+}
+
+// Bit p of the result: class bit K set in byte p of the 64 bytes c[0..15].
+template <uint32_t K>
+WJQ_HD uint64_t fmask(const uint32_t (&c)[16]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  constexpr uint32_t k = K == 1u ? 0 : K == 2u ? 1 : K == 4u ? 2 : K == 8u ? 3 : K == 16u ? 4 : K == 32u ? 5 : 6;
+  constexpr uint32_t B4 = K * 0x01010101u;
+  uint32_t h[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    // 16 bytes: two chained dot products per 8 bytes give (pattern << k)
+    const uint32_t v0 = __builtin_amdgcn_udot4(c[4 * q + 1] & B4, 0x80402010u,
+                                               __builtin_amdgcn_udot4(c[4 * q] & B4, 0x08040201u, 0u, false), false);
+    const uint32_t v1 = __builtin_amdgcn_udot4(c[4 * q + 3] & B4, 0x80402010u,
+                                               __builtin_amdgcn_udot4(c[4 * q + 2] & B4, 0x08040201u, 0u, false), false);
+    h[q] = ((v1 << 8) | v0) >> k;
+  }
+  return (uint64_t)(h[0] | (h[1] << 16)) | ((uint64_t)(h[2] | (h[3] << 16)) << 32);
+#else
+  uint64_t m = 0;
+  for (int q = 0; q < 16; q++)
+    for (int b = 0; b < 4; b++)
+      if ((c[q] >> (8 * b)) & K) m |= 1ull << (4 * q + b);
+  return m;
+#endif
+}
+
+// Bit p: parity of the bits of x at or below p.
+WJQ_HD uint64_t prefix_xor64(uint64_t x) {
+  x ^= x << 1;
+  x ^= x << 2;
+  x ^= x << 4;
+  x ^= x << 8;
+  x ^= x << 16;
+  x ^= x << 32;
+  return x;
+}
+
+// The first set bit at or after bit `from` of the FLAT_WORDS-word mask m
+// (position relative to word 0), or 64 * FLAT_WORDS when there is none.
+WJQ_HD uint32_t flat_next(const uint64_t (&m)[FLAT_WORDS], uint32_t from) {
+  uint32_t r = 64u * FLAT_WORDS;
+#pragma unroll
+  for (int w = FLAT_WORDS - 1; w >= 0; w--) {
+    const uint32_t base = 64u * (uint32_t)w;
+    uint64_t x = m[w];
+    if (from >= base + 64u) x = 0;
+    else if (from > base) x &= ~0ull << (from - base);
+    if (x) r = base + (uint32_t)ctz64(x);
+  }
+  return r;
+}
+
+template <class R, class = void>
+struct has_block_reads : std::false_type {};
+template <class R>
+struct has_block_reads<R, std::void_t<decltype(&R::block)>> : std::true_type {};
+
+// `i` is the line's '{'. True: the line is a valid flat object; the fused
+// slots, `te` and `end` are set as walk() sets them. False: nothing is
+// written; walk the line.
+template <int NS, class R, class PP>
+WJQ_HD bool flat_walk(R &rd, uint32_t i, const PP &P, SlotV (&sl)[NS], uint32_t &te, uint32_t &end) {
+  const uint32_t B = i & ~15u;
+  const uint32_t o0 = i - B;
+  uint64_t KSw[FLAT_WORDS], EDw[FLAT_WORDS], KEw[FLAT_WORDS];
+  uint64_t bad = 0;
+  uint32_t n_other = 0, str_c = 0, add_c = 0, bsl = 0;
+  uint32_t pQc = 0, pCM = 0, pCL = 0, pDG = 0, pOP = 0, pKE = 0, pZR = 0;
+  uint32_t e_pos = 0;
+  bool ended = false;
+#pragma unroll
+  for (int w = 0; w < FLAT_WORDS; w++) {
+    KSw[w] = EDw[w] = KEw[w] = 0;
+    if (ended) continue;
+    uint32_t c[16];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const Blk16 v = rd.block((B >> 4) + 4u * (uint32_t)w + (uint32_t)j);
+      c[4 * j] = fclass4(v.x);
+      c[4 * j + 1] = fclass4(v.y);
+      c[4 * j + 2] = fclass4(v.z);
+      c[4 * j + 3] = fclass4(v.w);
+    }
+    uint32_t orc = 0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) orc |= c[j];
+    bsl |= orc & (FC_BSL * 0x01010101u);  // a backslash anywhere near: walk it
+    const uint64_t lo = w == 0 ? (~0ull << o0) : ~0ull;
+    const uint64_t ct = fmask<FC_CTRL>(c) & lo;
+    uint64_t in = lo;
+    if (ct) {
+      const uint32_t e = (uint32_t)ctz64(ct);
+      in &= e ? (~0ull >> (64u - e)) : 0ull;
+      e_pos = B + 64u * (uint32_t)w + e;
+      ended = true;
+    }
+    const uint64_t q = fmask<FC_QUOTE>(c) & in;
+    const uint64_t S = (prefix_xor64(q) ^ (str_c ? ~0ull : 0ull)) & in;
+    str_c ^= (uint32_t)__builtin_popcountll(q) & 1u;
+    const uint64_t Qo = q & S, Qc = q & ~S;
+    const uint64_t out = in & ~S & ~q;
+    const uint64_t cm = fmask<FC_COMMA>(c) & out, cl = fmask<FC_COLON>(c) & out;
+    const uint64_t dg = fmask<FC_DIGIT>(c) & out, zr = fmask<FC_ZERO>(c) & out;
+    const uint64_t OP = w == 0 ? (1ull << o0) : 0ull;
+    const uint64_t oth = out & ~(cm | cl | dg | OP);  // must be the closing '}' alone
+    n_other += (uint32_t)__builtin_popcountll(oth);
+    const uint64_t PQc = (Qc << 1) | pQc, PCM = (cm << 1) | pCM, PCL = (cl << 1) | pCL;
+    const uint64_t PDG = (dg << 1) | pDG, POP = (OP << 1) | pOP;
+    const uint64_t KS = Qo & (PCM | POP), VS = Qo & PCL;
+    bad |= Qo & ~(KS | VS);
+    // each key string's run, carried to its closing quote
+    const uint64_t t = S + KS;
+    const uint32_t c1 = t < S ? 1u : 0u;
+    const uint64_t sum = t + add_c;
+    const uint32_t c2 = sum < t ? 1u : 0u;
+    add_c = c1 | c2;
+    const uint64_t KE = sum & ~S & in;
+    const uint64_t PKE = (KE << 1) | pKE;
+    bad |= (PQc & in) & ~(cl | cm | oth);
+    bad |= (PCL & in) & ~(Qo | dg);
+    bad |= (PCM & in) & ~Qo;
+    bad |= (POP & in) & ~(Qo | oth);
+    bad |= (PDG & in) & ~(dg | cm | oth);
+    bad |= cm & ~(PQc | PDG);
+    bad |= oth & ~(PQc | PDG | POP);
+    const uint64_t RS = dg & ~PDG;
+    bad |= RS & ~PCL;
+    const uint64_t ZRS = zr & RS;
+    bad |= (((ZRS << 1) | pZR) & dg);  // a leading zero followed by a digit
+    bad |= (PKE & in) ^ cl;            // colons are exactly the bytes after key ends
+    KSw[w] = KS;
+    KEw[w] = KE;
+    EDw[w] = Qc | cm | oth;
+    pQc = (uint32_t)(Qc >> 63);
+    pCM = (uint32_t)(cm >> 63);
+    pCL = (uint32_t)(cl >> 63);
+    pDG = (uint32_t)(dg >> 63);
+    pOP = (uint32_t)(OP >> 63);
+    pKE = (uint32_t)(KE >> 63);
+    pZR = (uint32_t)(ZRS >> 63);
+  }
+  if (!ended || bad || bsl || str_c || n_other != 1) return false;
+  if (rd.byte(e_pos) != '\n' || rd.byte(e_pos - 1) != '}') return false;
+  // valid: the root object's fused slots (walk's value_hits at the root)
+  uint32_t t_err = 0;
+#pragma unroll
+  for (int q = 0; q < NS; q++) {
+    sl[q].off = 0;
+    sl[q].len = 0;
+    sl[q].kind = JK_MISSING;
+    if (((P.fused_mask >> q) & 1) && P.steps[P.spath[q].step_off].kind != WARPJQ_STEP_KEY) t_err |= 1u << q;
+  }
+  uint32_t want = 0;  // fused slots whose first step is a key
+#pragma unroll
+  for (int q = 0; q < NS; q++)
+    if (((P.fused_mask >> q) & 1) && P.steps[P.spath[q].step_off].kind == WARPJQ_STEP_KEY) want |= 1u << q;
+  if (want) {
+#pragma unroll
+    for (int w = 0; w < FLAT_WORDS; w++) {
+      uint64_t ks = KSw[w];
+      while (ks) {
+        const uint32_t kq = 64u * (uint32_t)w + (uint32_t)ctz64(ks);  // opening quote, from B
+        ks &= ks - 1;
+        const uint32_t ke = flat_next(KEw, kq + 1u);  // closing quote
+        const uint32_t kl = ke - kq - 1u;
+        if (!((P.key_lens0 >> (kl < 31u ? kl : 31u)) & 1u)) continue;
+        const uint32_t ka = B + kq + 1u;
+        uint32_t hit = 0;
+        uint32_t A = want;
+        while (A) {
+          const uint32_t q = (uint32_t)ctz32(A);
+          A &= A - 1;
+          const auto &st = P.steps[P.spath[q].step_off];
+          if (st.key_len != kl) continue;
+          bool same = true;
+          for (uint32_t k = 0; k < kl && same; k++) same = rd.byte(ka + k) == P.blob[st.key_off + k];
+          if (same) hit |= 1u << q;
+        }
+        if (!hit) continue;
+        const uint32_t vs = ke + 2u;  // after '":'
+        const uint32_t ve0 = flat_next(EDw, vs + 1u);
+        const bool str = rd.byte(B + vs) == '"';
+        const uint32_t ve = str ? ve0 + 1u : ve0;
+#pragma unroll
+        for (int q = 0; q < NS; q++) {
+          if (!((hit >> q) & 1)) continue;
+          if (P.spath[q].nsteps == 1) {
+            sl[q].off = B + vs;
+            sl[q].len = ve - vs;
+            sl[q].kind = str ? (uint32_t)JK_STR : (uint32_t)JK_NUM;
+            t_err &= ~(1u << q);
+          } else {
+            sl[q].kind = JK_MISSING;  // a string or number indexed further
+            t_err |= 1u << q;
+          }
+        }
+      }
+    }
+  }
+  te = t_err;
+  end = e_pos;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
 // one line, in the CPU engine's order (cpu.rs:308-338)
 // ---------------------------------------------------------------------------
 
@@ -1389,7 +1659,12 @@ WJQ_HD LineResult eval_line(R &rd, uint32_t s, const PP &P, SlotV (&sl)[NS]) {
     sl[q].kind = JK_MISSING;
   }
   uint32_t te = 0;
-  int st = walk<NS>(rd, i, P, sl, te, r.end);
+  int st;
+  if constexpr (has_block_reads<R>::value && !wide_reader<R>::value) {
+    st = (rd.byte(i) == '{' && flat_walk<NS>(rd, i, P, sl, te, r.end)) ? (int)ST_OK : walk<NS>(rd, i, P, sl, te, r.end);
+  } else {
+    st = walk<NS>(rd, i, P, sl, te, r.end);
+  }
   if (st != ST_OK) {
     r.status = (uint32_t)st;
     return r;
