@@ -1366,8 +1366,8 @@ WJQ_HD int eval_cond(R &rd, const PP &P, const SlotV (&sl)[NS]) {
 //     carries each key's run up to its closing quote ((S + KS) & ~S), and
 //     those closing quotes must be exactly the bytes before the colons.
 // Any byte outside that shape (whitespace, escapes, signs, fractions,
-// literals, nesting, control bytes, a line longer than FLAT_WORDS x 64
-// bytes) makes it return false and the caller walks the line byte by byte,
+// literals, nesting, control bytes, a line running past the first
+// (FLAT_WORDS - 1) x 64 + 32 bytes from its 16-byte block) makes it return false and the caller walks the line byte by byte,
 // which decides everything; when it returns true the line is valid and the
 // captures of the fused paths are those walk() would take. Each lane runs it
 // on its own line, so there is no carry between lanes.
@@ -1414,15 +1414,16 @@ WJQ_HD uint32_t fclass4(uint32_t x) {
 #endif
 }
 
-// Bit p of the result: class bit K set in byte p of the 64 bytes c[0..15].
-template <uint32_t K>
+// Bit p of the result: class bit K set in byte p of the first NB x 16 bytes
+// of c[0..15] (bits past them zero).
+template <uint32_t K, int NB = 4>
 WJQ_HD uint64_t fmask(const uint32_t (&c)[16]) {
 #if defined(__HIP_DEVICE_COMPILE__)
   constexpr uint32_t k = K == 1u ? 0 : K == 2u ? 1 : K == 4u ? 2 : K == 8u ? 3 : K == 16u ? 4 : K == 32u ? 5 : 6;
   constexpr uint32_t B4 = K * 0x01010101u;
-  uint32_t h[4];
+  uint32_t h[4] = {0, 0, 0, 0};
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
+  for (int q = 0; q < NB; q++) {
     // 16 bytes: two chained dot products per 8 bytes give (pattern << k)
     const uint32_t v0 = __builtin_amdgcn_udot4(c[4 * q + 1] & B4, 0x80402010u,
                                                __builtin_amdgcn_udot4(c[4 * q] & B4, 0x08040201u, 0u, false), false);
@@ -1433,7 +1434,7 @@ WJQ_HD uint64_t fmask(const uint32_t (&c)[16]) {
   return (uint64_t)(h[0] | (h[1] << 16)) | ((uint64_t)(h[2] | (h[3] << 16)) << 32);
 #else
   uint64_t m = 0;
-  for (int q = 0; q < 16; q++)
+  for (int q = 0; q < 4 * NB; q++)
     for (int b = 0; b < 4; b++)
       if ((c[q] >> (8 * b)) & K) m |= 1ull << (4 * q + b);
   return m;
@@ -1474,141 +1475,213 @@ struct has_block_reads<R, std::void_t<decltype(&R::block)>> : std::true_type {};
 // `i` is the line's '{'. True: the line is a valid flat object; the fused
 // slots, `te` and `end` are set as walk() sets them. False: nothing is
 // written; walk the line.
-template <int NS, class R, class PP>
-WJQ_HD bool flat_walk(R &rd, uint32_t i, const PP &P, SlotV (&sl)[NS], uint32_t &te, uint32_t &end) {
-  const uint32_t B = i & ~15u;
-  const uint32_t o0 = i - B;
-  uint64_t KSw[FLAT_WORDS], EDw[FLAT_WORDS], KEw[FLAT_WORDS];
+// One 64-byte word (the last of a line's words holds 32 bytes: NB blocks)
+// of flat_walk: classes, masks and the neighbour relations, carrying the
+// string parity, the key-run addition and the top bits of the shifted masks
+// into the next word.
+struct FlatCarry {
   uint64_t bad = 0;
   uint32_t n_other = 0, str_c = 0, add_c = 0, bsl = 0;
   uint32_t pQc = 0, pCM = 0, pCL = 0, pDG = 0, pOP = 0, pKE = 0, pZR = 0;
   uint32_t e_pos = 0;
   bool ended = false;
+};
+
+template <int NB, class R>
+WJQ_HD void flat_word(R &rd, uint32_t B, uint32_t o0, uint32_t w, FlatCarry &F, uint64_t &KSo, uint64_t &KEo,
+                      uint64_t &EDo) {
+  uint32_t c[16];
 #pragma unroll
-  for (int w = 0; w < FLAT_WORDS; w++) {
-    KSw[w] = EDw[w] = KEw[w] = 0;
-    if (ended) continue;
-    uint32_t c[16];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const Blk16 v = rd.block((B >> 4) + 4u * (uint32_t)w + (uint32_t)j);
+  for (int j = 0; j < 4; j++) {
+    if (j < NB) {
+      const Blk16 v = rd.block((B >> 4) + 4u * w + (uint32_t)j);
       c[4 * j] = fclass4(v.x);
       c[4 * j + 1] = fclass4(v.y);
       c[4 * j + 2] = fclass4(v.z);
       c[4 * j + 3] = fclass4(v.w);
+    } else {
+      c[4 * j] = c[4 * j + 1] = c[4 * j + 2] = c[4 * j + 3] = 0;
     }
-    uint32_t orc = 0;
+  }
+  uint32_t orc = 0;
 #pragma unroll
-    for (int j = 0; j < 16; j++) orc |= c[j];
-    bsl |= orc & (FC_BSL * 0x01010101u);  // a backslash anywhere near: walk it
-    const uint64_t lo = w == 0 ? (~0ull << o0) : ~0ull;
-    const uint64_t ct = fmask<FC_CTRL>(c) & lo;
-    uint64_t in = lo;
+  for (int j = 0; j < 4 * NB; j++) orc |= c[j];
+  F.bsl |= orc & (FC_BSL * 0x01010101u);  // a backslash anywhere near: walk it
+  const uint64_t lo = w == 0 ? (~0ull << o0) : ~0ull;
+  uint64_t in = lo;
+  if (orc & (FC_CTRL * 0x01010101u)) {  // words with no control byte skip this mask
+    const uint64_t ct = fmask<FC_CTRL, NB>(c) & lo;
     if (ct) {
       const uint32_t e = (uint32_t)ctz64(ct);
       in &= e ? (~0ull >> (64u - e)) : 0ull;
-      e_pos = B + 64u * (uint32_t)w + e;
-      ended = true;
+      F.e_pos = B + 64u * w + e;
+      F.ended = true;
     }
-    const uint64_t q = fmask<FC_QUOTE>(c) & in;
-    const uint64_t S = (prefix_xor64(q) ^ (str_c ? ~0ull : 0ull)) & in;
-    str_c ^= (uint32_t)__builtin_popcountll(q) & 1u;
-    const uint64_t Qo = q & S, Qc = q & ~S;
-    const uint64_t out = in & ~S & ~q;
-    const uint64_t cm = fmask<FC_COMMA>(c) & out, cl = fmask<FC_COLON>(c) & out;
-    const uint64_t dg = fmask<FC_DIGIT>(c) & out, zr = fmask<FC_ZERO>(c) & out;
-    const uint64_t OP = w == 0 ? (1ull << o0) : 0ull;
-    const uint64_t oth = out & ~(cm | cl | dg | OP);  // must be the closing '}' alone
-    n_other += (uint32_t)__builtin_popcountll(oth);
-    const uint64_t PQc = (Qc << 1) | pQc, PCM = (cm << 1) | pCM, PCL = (cl << 1) | pCL;
-    const uint64_t PDG = (dg << 1) | pDG, POP = (OP << 1) | pOP;
-    const uint64_t KS = Qo & (PCM | POP), VS = Qo & PCL;
-    bad |= Qo & ~(KS | VS);
-    // each key string's run, carried to its closing quote
-    const uint64_t t = S + KS;
-    const uint32_t c1 = t < S ? 1u : 0u;
-    const uint64_t sum = t + add_c;
-    const uint32_t c2 = sum < t ? 1u : 0u;
-    add_c = c1 | c2;
-    const uint64_t KE = sum & ~S & in;
-    const uint64_t PKE = (KE << 1) | pKE;
-    bad |= (PQc & in) & ~(cl | cm | oth);
-    bad |= (PCL & in) & ~(Qo | dg);
-    bad |= (PCM & in) & ~Qo;
-    bad |= (POP & in) & ~(Qo | oth);
-    bad |= (PDG & in) & ~(dg | cm | oth);
-    bad |= cm & ~(PQc | PDG);
-    bad |= oth & ~(PQc | PDG | POP);
-    const uint64_t RS = dg & ~PDG;
-    bad |= RS & ~PCL;
-    const uint64_t ZRS = zr & RS;
-    bad |= (((ZRS << 1) | pZR) & dg);  // a leading zero followed by a digit
-    bad |= (PKE & in) ^ cl;            // colons are exactly the bytes after key ends
-    KSw[w] = KS;
-    KEw[w] = KE;
-    EDw[w] = Qc | cm | oth;
-    pQc = (uint32_t)(Qc >> 63);
-    pCM = (uint32_t)(cm >> 63);
-    pCL = (uint32_t)(cl >> 63);
-    pDG = (uint32_t)(dg >> 63);
-    pOP = (uint32_t)(OP >> 63);
-    pKE = (uint32_t)(KE >> 63);
-    pZR = (uint32_t)(ZRS >> 63);
   }
-  if (!ended || bad || bsl || str_c || n_other != 1) return false;
+  if (NB < 4) in &= (1ull << (16 * NB)) - 1;
+  const uint64_t q = fmask<FC_QUOTE, NB>(c) & in;
+  const uint64_t S = (prefix_xor64(q) ^ (F.str_c ? ~0ull : 0ull)) & in;
+  F.str_c ^= (uint32_t)__builtin_popcountll(q) & 1u;
+  const uint64_t Qo = q & S, Qc = q & ~S;
+  const uint64_t out = in & ~S & ~q;
+  const uint64_t cm = fmask<FC_COMMA, NB>(c) & out, cl = fmask<FC_COLON, NB>(c) & out;
+  const uint64_t dg = fmask<FC_DIGIT, NB>(c) & out, zr = fmask<FC_ZERO, NB>(c) & out;
+  const uint64_t OP = w == 0 ? (1ull << o0) : 0ull;
+  const uint64_t oth = out & ~(cm | cl | dg | OP);  // must be the closing '}' alone
+  F.n_other += (uint32_t)__builtin_popcountll(oth);
+  const uint64_t PQc = (Qc << 1) | F.pQc, PCM = (cm << 1) | F.pCM, PCL = (cl << 1) | F.pCL;
+  const uint64_t PDG = (dg << 1) | F.pDG, POP = (OP << 1) | F.pOP;
+  const uint64_t KS = Qo & (PCM | POP), VS = Qo & PCL;
+  uint64_t bad = Qo & ~(KS | VS);
+  // each key string's run, carried to its closing quote
+  const uint64_t t = S + KS;
+  const uint32_t c1 = t < S ? 1u : 0u;
+  const uint64_t sum = t + F.add_c;
+  const uint32_t c2 = sum < t ? 1u : 0u;
+  F.add_c = c1 | c2;
+  const uint64_t KE = sum & ~S & in;
+  const uint64_t PKE = (KE << 1) | F.pKE;
+  bad |= (PQc & in) & ~(cl | cm | oth);
+  bad |= (PCL & in) & ~(Qo | dg);
+  bad |= (PCM & in) & ~Qo;
+  bad |= (POP & in) & ~(Qo | oth);
+  bad |= (PDG & in) & ~(dg | cm | oth);
+  bad |= cm & ~(PQc | PDG);
+  bad |= oth & ~(PQc | PDG | POP);
+  const uint64_t RS = dg & ~PDG;
+  bad |= RS & ~PCL;
+  const uint64_t ZRS = zr & RS;
+  bad |= (((ZRS << 1) | F.pZR) & dg);  // a leading zero followed by a digit
+  bad |= (PKE & in) ^ cl;              // colons are exactly the bytes after key ends
+  F.bad |= bad;
+  KSo = KS;
+  KEo = KE;
+  EDo = Qc | cm | oth;
+  F.pQc = (uint32_t)(Qc >> 63);
+  F.pCM = (uint32_t)(cm >> 63);
+  F.pCL = (uint32_t)(cl >> 63);
+  F.pDG = (uint32_t)(dg >> 63);
+  F.pOP = (uint32_t)(OP >> 63);
+  F.pKE = (uint32_t)(KE >> 63);
+  F.pZR = (uint32_t)(ZRS >> 63);
+}
+
+// Bits of the FLAT_WORDS-word mask m shifted down by k (0 < k < 64), word w.
+WJQ_HD uint64_t flat_shr(const uint64_t (&m)[FLAT_WORDS], int w, uint32_t k) {
+  const uint64_t hi = w + 1 < FLAT_WORDS ? m[w + 1] : 0ull;
+  return (m[w] >> k) | (hi << (64u - k));
+}
+
+// Records the value after key `kq` (its opening quote, from B; `kl` bytes
+// long) for the fused slots in `hit`.
+template <int NS, class R, class PP>
+WJQ_HD void flat_take(R &rd, const PP &P, uint32_t B, uint32_t ke, uint32_t hit, const uint64_t (&EDw)[FLAT_WORDS],
+                      SlotV (&sl)[NS], uint32_t &t_err) {
+  const uint32_t vs = ke + 2u;  // after '":'
+  const uint32_t ve0 = flat_next(EDw, vs + 1u);
+  const bool str = rd.byte(B + vs) == '"';
+  const uint32_t ve = str ? ve0 + 1u : ve0;
+#pragma unroll
+  for (int q = 0; q < NS; q++) {
+    if (!((hit >> q) & 1)) continue;
+    if (P.spath[q].nsteps == 1) {
+      sl[q].off = B + vs;
+      sl[q].len = ve - vs;
+      sl[q].kind = str ? (uint32_t)JK_STR : (uint32_t)JK_NUM;
+      t_err &= ~(1u << q);
+    } else {
+      sl[q].kind = JK_MISSING;  // a string or number indexed further
+      t_err |= 1u << q;
+    }
+  }
+}
+
+// Fused slots among `want` whose first step is the key of `kl` bytes at `ka`.
+template <int NS, class R, class PP>
+WJQ_HD uint32_t flat_key_hits(R &rd, const PP &P, uint32_t want, uint32_t ka, uint32_t kl) {
+  uint32_t hit = 0;
+  uint32_t A = want;
+  while (A) {
+    const uint32_t q = (uint32_t)ctz32(A);
+    A &= A - 1;
+    const auto &st = P.steps[P.spath[q].step_off];
+    if (st.key_len != kl) continue;
+    bool same = true;
+    for (uint32_t k = 0; k < kl && same; k++) same = rd.byte(ka + k) == P.blob[st.key_off + k];
+    if (same) hit |= 1u << q;
+  }
+  return hit;
+}
+
+// `i` is the line's '{'. True: the line is a valid flat object; the fused
+// slots, `te` and `end` are set as walk() sets them. False: nothing is
+// written; walk the line.
+template <int NS, class R, class PP>
+WJQ_HD bool flat_walk(R &rd, uint32_t i, const PP &P, SlotV (&sl)[NS], uint32_t &te, uint32_t &end) {
+  const uint32_t B = i & ~15u;
+  const uint32_t o0 = i - B;
+  uint64_t KSw[FLAT_WORDS], EDw[FLAT_WORDS], KEw[FLAT_WORDS];
+  FlatCarry F;
+#pragma unroll
+  for (int w = 0; w < FLAT_WORDS; w++) {
+    KSw[w] = EDw[w] = KEw[w] = 0;
+    if (F.ended) continue;
+    if (w + 1 < FLAT_WORDS) flat_word<4>(rd, B, o0, (uint32_t)w, F, KSw[w], KEw[w], EDw[w]);
+    else flat_word<2>(rd, B, o0, (uint32_t)w, F, KSw[w], KEw[w], EDw[w]);
+  }
+  if (!F.ended || F.bad || F.bsl || F.str_c || F.n_other != 1) return false;
+  const uint32_t e_pos = F.e_pos;
   if (rd.byte(e_pos) != '\n' || rd.byte(e_pos - 1) != '}') return false;
   // valid: the root object's fused slots (walk's value_hits at the root)
-  uint32_t t_err = 0;
+  uint32_t t_err = 0, want = 0;  // want: fused slots whose first step is a key
 #pragma unroll
   for (int q = 0; q < NS; q++) {
     sl[q].off = 0;
     sl[q].len = 0;
     sl[q].kind = JK_MISSING;
-    if (((P.fused_mask >> q) & 1) && P.steps[P.spath[q].step_off].kind != WARPJQ_STEP_KEY) t_err |= 1u << q;
+    if ((P.fused_mask >> q) & 1) {
+      if (P.steps[P.spath[q].step_off].kind == WARPJQ_STEP_KEY) want |= 1u << q;
+      else t_err |= 1u << q;
+    }
   }
-  uint32_t want = 0;  // fused slots whose first step is a key
-#pragma unroll
-  for (int q = 0; q < NS; q++)
-    if (((P.fused_mask >> q) & 1) && P.steps[P.spath[q].step_off].kind == WARPJQ_STEP_KEY) want |= 1u << q;
   if (want) {
+    if (!(P.key_lens0 >> 31)) {
+      // every wanted key is shorter than 31 bytes: candidates by length,
+      // key starts whose key end lies len + 1 bytes on (KS & KE >> (len + 1)),
+      // in line order (a duplicate key's later value wins)
+      uint64_t cand[FLAT_WORDS];
 #pragma unroll
-    for (int w = 0; w < FLAT_WORDS; w++) {
-      uint64_t ks = KSw[w];
-      while (ks) {
-        const uint32_t kq = 64u * (uint32_t)w + (uint32_t)ctz64(ks);  // opening quote, from B
-        ks &= ks - 1;
-        const uint32_t ke = flat_next(KEw, kq + 1u);  // closing quote
-        const uint32_t kl = ke - kq - 1u;
-        if (!((P.key_lens0 >> (kl < 31u ? kl : 31u)) & 1u)) continue;
-        const uint32_t ka = B + kq + 1u;
-        uint32_t hit = 0;
-        uint32_t A = want;
-        while (A) {
-          const uint32_t q = (uint32_t)ctz32(A);
-          A &= A - 1;
-          const auto &st = P.steps[P.spath[q].step_off];
-          if (st.key_len != kl) continue;
-          bool same = true;
-          for (uint32_t k = 0; k < kl && same; k++) same = rd.byte(ka + k) == P.blob[st.key_off + k];
-          if (same) hit |= 1u << q;
+      for (int w = 0; w < FLAT_WORDS; w++) cand[w] = 0;
+      uint32_t lens = P.key_lens0;
+      while (lens) {
+        const uint32_t L = (uint32_t)ctz32(lens);
+        lens &= lens - 1;
+#pragma unroll
+        for (int w = 0; w < FLAT_WORDS; w++) cand[w] |= KSw[w] & flat_shr(KEw, w, L + 1u);
+      }
+#pragma unroll
+      for (int w = 0; w < FLAT_WORDS; w++) {
+        uint64_t cb = cand[w];
+        while (cb) {
+          const uint32_t kq = 64u * (uint32_t)w + (uint32_t)ctz64(cb);
+          cb &= cb - 1;
+          const uint32_t ke = flat_next(KEw, kq + 1u);
+          const uint32_t hit = flat_key_hits<NS>(rd, P, want, B + kq + 1u, ke - kq - 1u);
+          if (hit) flat_take<NS>(rd, P, B, ke, hit, EDw, sl, t_err);
         }
-        if (!hit) continue;
-        const uint32_t vs = ke + 2u;  // after '":'
-        const uint32_t ve0 = flat_next(EDw, vs + 1u);
-        const bool str = rd.byte(B + vs) == '"';
-        const uint32_t ve = str ? ve0 + 1u : ve0;
+      }
+    } else {
 #pragma unroll
-        for (int q = 0; q < NS; q++) {
-          if (!((hit >> q) & 1)) continue;
-          if (P.spath[q].nsteps == 1) {
-            sl[q].off = B + vs;
-            sl[q].len = ve - vs;
-            sl[q].kind = str ? (uint32_t)JK_STR : (uint32_t)JK_NUM;
-            t_err &= ~(1u << q);
-          } else {
-            sl[q].kind = JK_MISSING;  // a string or number indexed further
-            t_err |= 1u << q;
-          }
+      for (int w = 0; w < FLAT_WORDS; w++) {
+        uint64_t ks = KSw[w];
+        while (ks) {
+          const uint32_t kq = 64u * (uint32_t)w + (uint32_t)ctz64(ks);  // opening quote, from B
+          ks &= ks - 1;
+          const uint32_t ke = flat_next(KEw, kq + 1u);  // closing quote
+          const uint32_t kl = ke - kq - 1u;
+          if (!((P.key_lens0 >> (kl < 31u ? kl : 31u)) & 1u)) continue;
+          const uint32_t hit = flat_key_hits<NS>(rd, P, want, B + kq + 1u, kl);
+          if (hit) flat_take<NS>(rd, P, B, ke, hit, EDw, sl, t_err);
         }
       }
     }
