@@ -71,7 +71,8 @@ struct warpjq_amd_server {
   uint64_t budget = 0, reserved = 0, chunk_bytes = 0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::deque<Dataset> ds;  // stable addresses: workers read entries unlocked
+  std::deque<Dataset> ds;  // elements keep their addresses; the deque itself is read under `mu`
+  std::vector<uint8_t *> to_free;  // evicted device buffers, freed by a worker outside `mu`
   std::deque<Request *> waiting;
   std::map<int64_t, std::unique_ptr<Request>> reqs;
   int64_t next_id = 1;
@@ -106,7 +107,8 @@ bool make_room(warpjq_amd_server &s, uint64_t need, const Request &r) {
       if (!victim || d.last_use < victim->last_use) victim = &d;
     }
     if (!victim) return false;
-    (void)hipFree(victim->dev);
+    // freed after the lock is released: hipFree waits for the device
+    s.to_free.push_back(victim->dev);
     victim->dev = nullptr;
     s.reserved -= victim->charge();
     s.evictions++;
@@ -163,6 +165,7 @@ void worker(warpjq_amd_server *s) {
   uint64_t uses = 0;
   for (;;) {
     Request *r = nullptr;
+    std::vector<uint8_t *> evicted;
     {
       std::unique_lock<std::mutex> lk(s->mu);
       for (;;) {
@@ -171,20 +174,28 @@ void worker(warpjq_amd_server *s) {
         if (r) break;
         s->cv_work.wait(lk);
       }
-      if (!r) break;
+      evicted.swap(s->to_free);
+      if (!r) {
+        lk.unlock();
+        for (uint8_t *p : evicted) (void)hipFree(p);
+        break;
+      }
     }
-    // load what this request needs and another request has not
-    std::vector<uint32_t> mine;
+    for (uint8_t *p : evicted) (void)hipFree(p);
+    // load what this request needs and another request has not; the
+    // entries' addresses are taken under the lock (add_dataset may grow the
+    // deque meanwhile) and their host bytes and sizes never change
+    std::vector<Dataset *> mine;
     {
       std::lock_guard<std::mutex> g(s->mu);
-      for (uint32_t i : r->ds)
-        if (s->ds[i].loading && !s->ds[i].dev &&
-            std::find(mine.begin(), mine.end(), i) == mine.end())
-          mine.push_back(i);
+      for (uint32_t i : r->ds) {
+        Dataset *d = &s->ds[i];
+        if (d->loading && !d->dev && std::find(mine.begin(), mine.end(), d) == mine.end()) mine.push_back(d);
+      }
     }
     std::string lerr;
-    for (uint32_t i : mine) {
-      Dataset &d = s->ds[i];
+    for (Dataset *dptr : mine) {
+      Dataset &d = *dptr;
       uint8_t *p = nullptr;
       hipError_t e = hipMalloc((void **)&p, d.charge());
       if (e == hipSuccess && d.n) e = hipMemcpy(p, d.host, d.n, hipMemcpyHostToDevice);
@@ -315,6 +326,7 @@ void warpjq_amd_server_destroy(warpjq_amd_server *s) {
     s->cv_work.notify_all();
   }
   for (auto &t : s->workers) t.join();
+  for (uint8_t *p : s->to_free) (void)hipFree(p);
   for (auto &kv : s->reqs) free(kv.second->out);
   for (auto &d : s->ds)
     if (d.dev) (void)hipFree(d.dev);
