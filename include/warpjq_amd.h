@@ -236,6 +236,26 @@ int warpjq_amd_gen_nginx(uint64_t seed, uint64_t target_bytes, uint32_t threads,
 int warpjq_amd_gen_nested(uint64_t seed, uint64_t target_bytes, uint32_t threads, uint64_t pad, uint8_t **out,
                           uint64_t *n_bytes, uint64_t *n_lines, uint64_t *n_malformed);
 
+/* An ungrouped aggregate (count, sum, min, max, avg, with any filter) over
+ * n_bytes already in device memory that allocates nothing: the result of
+ * warpjq_ctx_create + warpjq_amd_submit_device + warpjq_wait for the same
+ * program and bytes, with every device buffer those kernels use besides the
+ * input kept in a fixed array that each GPU holds for the life of the
+ * process (8 MiB; warpjq_amd_noalloc_max_bytes() gives the largest input it
+ * serves, about 2.7 GiB; larger inputs are refused with
+ * WARPJQ_ERR_INVALID_ARG). No device memory is allocated or freed per call;
+ * the host side only builds the program's small tables. Calls on one GPU
+ * share that array and must not overlap (concurrent calls race); the
+ * caller's device must be current. Declined lines are counted in
+ * n_fallback without their offsets. `ext` (optional) receives the exact sum
+ * parts. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * hosts that forbid allocation after start-up). */
+uint64_t warpjq_amd_noalloc_max_bytes(void);
+warpjq_status warpjq_amd_aggregate_device_noalloc(const warpjq_program *prog, const uint8_t *d_bytes, uint64_t n_bytes,
+                                                  warpjq_chunk_result *out, warpjq_amd_agg_ext *ext, char *err,
+                                                  size_t err_len);
+
 #ifdef __cplusplus
 }
 #endif

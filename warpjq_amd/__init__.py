@@ -110,7 +110,8 @@ ADDITIVE_SYMBOLS = ["warpjq_amd_ext_value", "warpjq_amd_wait_ext", "warpjq_amd_s
                     "warpjq_amd_ctx_reprogram", "warpjq_amd_set_graphs", "warpjq_amd_autotune",
                     "warpjq_amd_free", "warpjq_amd_compile", "warpjq_amd_program_view",
                     "warpjq_amd_program_free", "warpjq_amd_gen_nginx", "warpjq_amd_gen_nested",
-                    "warpjq_amd_first_type_error"]
+                    "warpjq_amd_first_type_error", "warpjq_amd_noalloc_max_bytes",
+                    "warpjq_amd_aggregate_device_noalloc"]
 
 
 def build(jobs=8):
@@ -172,6 +173,11 @@ def lib():
                                           P(ctypes.c_float), cp, sz]
         L.warpjq_amd_device_upload.argtypes = [ctypes.c_void_p, u64, cp, sz]
         L.warpjq_amd_device_upload.restype = ctypes.c_void_p
+        L.warpjq_amd_noalloc_max_bytes.argtypes = []
+        L.warpjq_amd_noalloc_max_bytes.restype = u64
+        L.warpjq_amd_aggregate_device_noalloc.argtypes = [P(Program), ctypes.c_void_p, u64, P(ChunkResult),
+                                                          P(AggExt), cp, sz]
+        L.warpjq_amd_aggregate_device_noalloc.restype = ctypes.c_int
         L.warpjq_amd_device_free.argtypes = [ctypes.c_void_p]
         L.warpjq_amd_device_free.restype = None
         L.warpjq_amd_server_create.argtypes = [u64, u32, u64, cp, sz]
@@ -502,6 +508,20 @@ class TuneCache:
         with open(tmp, "w") as f:
             json.dump(self.d, f, indent=1, sort_keys=True)
         os.replace(tmp, self.path)
+
+
+def aggregate_device_noalloc(query, dptr, n):
+    """An ungrouped aggregate over n device bytes through
+    warpjq_amd_aggregate_device_noalloc (no device allocation per call).
+    Returns (ChunkResult, AggExt)."""
+    prog = Compiled(query)
+    r = ChunkResult()
+    x = AggExt()
+    e = _err()
+    rc = lib().warpjq_amd_aggregate_device_noalloc(prog.view, dptr, n, ctypes.byref(r), ctypes.byref(x), e, 512)
+    if rc != 0:
+        raise DeviceError(f"aggregate_device_noalloc: {STATUS_NAMES.get(rc, rc)}: {e.value.decode()}")
+    return r, x
 
 
 class DeviceBuffer:

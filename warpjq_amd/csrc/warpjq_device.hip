@@ -2,6 +2,8 @@
 // (gfx950): the non-template kernels, and the host side of the ABI. The
 // per-line kernels live in kernels.h and are instantiated per slot-table
 // width in kern_<N>.hip.
+#include <mutex>
+
 #include "kernels.h"
 
 // ---------------------------------------------------------------------------
@@ -1403,6 +1405,248 @@ warpjq_status warpjq_amd_set_graphs(warpjq_ctx *ctx, uint32_t per_slot) {
       s.graphs.erase(s.graphs.begin());
     }
   }
+  return WARPJQ_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// the no-allocation entry point
+//
+// warpjq_amd_aggregate_device_noalloc runs an ungrouped aggregate over bytes
+// already in device memory with the same kernels as warpjq_amd_submit_device
+// + warpjq_wait, but every buffer those kernels need beyond the input and the
+// result (counters, program tables, the span, per-wave partials, per-claim
+// line counts, the wave-wide phase's queue, declined-line records) lives in
+// `scratch`, a fixed array of NOALLOC_SCRATCH bytes that each GPU holds for
+// the life of the process. No device memory is allocated or freed per call.
+// Every call on one GPU uses that GPU's copy, and what the previous call left
+// there is cleared (by kernels) before use; calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+
+#define NOALLOC_SCRATCH (8u << 20)
+#define NOALLOC_FB 4096u
+#define NOALLOC_DEVICES 64
+
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH];
+
+namespace {
+
+std::mutex g_na_mu;
+uint8_t *g_na_ptr[NOALLOC_DEVICES];  // the device address of `scratch`, per device
+
+struct NaLayout {
+  size_t ctr, aggout, tab, span, parts, fb_idx, fb_off, fb_len, fb_tile, tile, longq, end;
+};
+
+NaLayout na_layout(uint32_t ntiles, uint32_t nparts) {
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  NaLayout L;
+  L.ctr = 0;
+  L.aggout = up(L.ctr + sizeof(DevCtr));
+  L.tab = up(L.aggout + sizeof(AggOut));
+  L.span = up(L.tab + TAB_LDS_MAX);
+  L.parts = up(L.span + sizeof(Span));
+  L.fb_idx = up(L.parts + (size_t)nparts * sizeof(AggPart));
+  L.fb_off = up(L.fb_idx + NOALLOC_FB * 4u);
+  L.fb_len = up(L.fb_off + NOALLOC_FB * 4u);
+  L.fb_tile = up(L.fb_len + NOALLOC_FB * 4u);
+  L.tile = up(L.fb_tile + NOALLOC_FB * 4u);
+  L.longq = up(L.tile + (size_t)ntiles * 8u);
+  L.end = up(L.longq + (size_t)ntiles * 12u);
+  return L;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+}  // namespace
+
+// The device address of this device's `scratch`, looked up once per device
+// (with that device current) and kept: host code never passes the array's
+// own name to a kernel.
+static warpjq_status noalloc_scratch(uint8_t **p, char *err, size_t err_len) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= NOALLOC_DEVICES) {
+    set_err(err, err_len, "aggregate_device_noalloc", "device index out of range");
+    return WARPJQ_ERR_INVALID_ARG;
+  }
+  std::lock_guard<std::mutex> g(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void *q = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_na_ptr[dev] = (uint8_t *)q;
+  }
+  *p = g_na_ptr[dev];
+  return WARPJQ_OK;
+}
+
+extern "C" {
+
+uint64_t warpjq_amd_noalloc_max_bytes(void) {
+  // the largest input whose per-claim arrays fit next to the fixed part
+  const NaLayout L0 = na_layout(0, 256u * MAX_WG_PER_CU);
+  const uint64_t per_tile = 20;
+  const uint64_t tiles = (NOALLOC_SCRATCH - L0.end - 512) / per_tile;
+  return tiles * CLAIM_DEFAULT - 16;
+}
+
+warpjq_status warpjq_amd_aggregate_device_noalloc(const warpjq_program *prog, const uint8_t *d_bytes, uint64_t n_bytes,
+                                                  warpjq_chunk_result *out, warpjq_amd_agg_ext *ext, char *err,
+                                                  size_t err_len) {
+  if (!prog || !out || (!d_bytes && n_bytes)) return WARPJQ_ERR_INVALID_ARG;
+  memset(out, 0, sizeof *out);
+  out->first_invalid = 0xFFFFFFFFu;
+  if (prog->output_kind != WARPJQ_OUT_AGG || prog->group_path != 0xFFFFFFFFu) {
+    set_err(err, err_len, "aggregate_device_noalloc", "only ungrouped aggregate queries (count, sum, min, max, avg)");
+    return WARPJQ_ERR_INVALID_ARG;
+  }
+  if (n_bytes > warpjq_amd_noalloc_max_bytes()) {
+    set_err(err, err_len, "aggregate_device_noalloc", "input larger than the fixed device scratch allows");
+    return WARPJQ_ERR_INVALID_ARG;
+  }
+  DProgHost host;
+  std::string why;
+  if (!build_dprog(*prog, host, why)) {
+    set_err(err, err_len, "query is not supported by the kernels", why.c_str());
+    return WARPJQ_ERR_INVALID_ARG;
+  }
+  // the program tables, laid out as put_program lays them
+  std::vector<uint8_t> buf;
+  size_t o_steps = put_table(buf, prog->steps, prog->n_steps);
+  size_t o_cmps = put_table(buf, prog->cmps, prog->n_cmps);
+  size_t o_cond = put_table(buf, host.cond.data(), host.cond.size());
+  size_t o_blob = put_table(buf, prog->blob, prog->n_blob);
+  size_t o_spath = put_table(buf, host.spath.data(), host.spath.size());
+  size_t o_sop = put_table(buf, prog->slot_of_path, prog->n_paths);
+  size_t o_fs = put_table(buf, host.field_slot.data(), host.field_slot.size());
+  size_t o_po = put_table(buf, prog->prefix_off, prog->n_fields);
+  size_t o_pl = put_table(buf, prog->prefix_len, prog->n_fields);
+  buf.resize((buf.size() + 15) & ~(size_t)15);
+  if (buf.size() > TAB_LDS_MAX) {
+    set_err(err, err_len, "query is too large for the kernels", "its tables exceed the 16 KiB staged in LDS");
+    return WARPJQ_ERR_INVALID_ARG;
+  }
+  uint8_t *S = nullptr;
+  if (warpjq_status st = noalloc_scratch(&S, err, err_len)) return st;
+  if (!g_num_cus) {
+    hipDeviceProp_t p;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (hipGetDeviceProperties(&p, dev) == hipSuccess) g_num_cus = (uint32_t)p.multiProcessorCount;
+    if (!g_num_cus) g_num_cus = 256;
+  }
+  const uint32_t claim = CLAIM_DEFAULT, arena = ARENA_DEFAULT;
+  const uint32_t tab_bytes = (uint32_t)buf.size();
+  const uint32_t lds = main_lds_bytes(arena, tab_bytes);
+  int occ;
+  switch (host.ns) {
+    case 1: occ = occupancy_main_1(false, false, lds); break;
+    case 2: occ = occupancy_main_2(false, false, lds); break;
+    case 4: occ = occupancy_main_4(false, false, lds); break;
+    case 8: occ = occupancy_main_8(false, false, lds); break;
+    default: occ = occupancy_main_16(false, false, lds); break;
+  }
+  occ = std::max(1, std::min(occ, (int)MAX_WG_PER_CU));
+  const uint32_t mis = (uint32_t)((uintptr_t)d_bytes & 15u);
+  const uint64_t vend = n_bytes ? mis + n_bytes : 0;
+  const uint32_t ntiles = (uint32_t)((vend + claim - 1) / claim);
+  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(g_num_cus * (uint32_t)occ, ntiles));
+  const NaLayout L = na_layout(ntiles, g_num_cus * (uint32_t)occ);
+  if (L.end > NOALLOC_SCRATCH) {
+    set_err(err, err_len, "aggregate_device_noalloc", "input larger than the fixed device scratch allows");
+    return WARPJQ_ERR_INVALID_ARG;
+  }
+  hipStream_t st = 0;
+  Span sp{d_bytes ? d_bytes - mis : nullptr, 0, mis, (uint32_t)vend, 0};
+  HIP_TRY(hipMemcpyAsync(S + L.tab, buf.data(), buf.size(), hipMemcpyHostToDevice, st), "H2D(program)");
+  HIP_TRY(hipMemcpyAsync(S + L.span, &sp, sizeof sp, hipMemcpyHostToDevice, st), "H2D(span)");
+  DevCtr *ctr = (DevCtr *)(S + L.ctr);
+  unsigned long long *tile = (unsigned long long *)(S + L.tile);
+  hipLaunchKernelGGL(k_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (ntiles + 255) / 256))), dim3(256), 0,
+                     st, ctr, tile, ntiles);
+  // the wave-wide phase's queue starts zero (its readers clear what they take)
+  hipLaunchKernelGGL(k_fill_u32, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (3u * ntiles + 255) / 256))),
+                     dim3(256), 0, st, (uint32_t *)(S + L.longq), (uint64_t)ntiles * 3u, 0u);
+  HIP_TRY(hipGetLastError(), "k_reset");
+  if (n_bytes) {
+    const uint8_t *tab = S + L.tab;
+    DProg d = host.d;
+    d.steps = (const warpjq_step *)(tab + o_steps);
+    d.cmps = (const warpjq_cmp *)(tab + o_cmps);
+    d.cond = (const warpjq_cond_op *)(tab + o_cond);
+    d.blob = tab + o_blob;
+    d.spath = (const SlotPath *)(tab + o_spath);
+    d.slot_of_path = (const uint32_t *)(tab + o_sop);
+    d.field_slot = (const uint32_t *)(tab + o_fs);
+    d.prefix_off = (const uint32_t *)(tab + o_po);
+    d.prefix_len = (const uint32_t *)(tab + o_pl);
+    KArgs a{};
+    a.spans = (const Span *)(S + L.span);
+    a.n_spans = 1;
+    a.ntiles = ntiles;
+    a.claim = claim;
+    a.arena = arena;
+    a.P = d;
+    a.tab = tab;
+    a.tab_bytes = tab_bytes;
+    a.ctr = ctr;
+    a.tile_state = tile;
+    a.parts = (AggPart *)(S + L.parts);
+    a.max_lines = 0;
+    a.fb_idx = (uint32_t *)(S + L.fb_idx);
+    a.fb_off = (uint32_t *)(S + L.fb_off);
+    a.fb_len = (uint32_t *)(S + L.fb_len);
+    a.fb_tile = (uint32_t *)(S + L.fb_tile);
+    a.fb_cap = NOALLOC_FB;
+    a.long_q = (uint32_t *)(S + L.longq);
+    a.long_cap = ntiles;
+    a.deep_cap = 0;
+    switch (host.ns) {
+      case 1: launch_main_1(false, false, grid, st, a); break;
+      case 2: launch_main_2(false, false, grid, st, a); break;
+      case 4: launch_main_4(false, false, grid, st, a); break;
+      case 8: launch_main_8(false, false, grid, st, a); break;
+      default: launch_main_16(false, false, grid, st, a); break;
+    }
+    HIP_TRY(hipGetLastError(), "k_main");
+    hipLaunchKernelGGL(k_agg_final, dim3(1), dim3(256), 0, st, a.parts, grid, ctr, (AggOut *)(S + L.aggout));
+    HIP_TRY(hipGetLastError(), "k_agg_final");
+  }
+  DevCtr c;
+  HIP_TRY(hipMemcpyAsync(&c, ctr, sizeof c, hipMemcpyDeviceToHost, st), "D2H(counters)");
+  HIP_TRY(hipStreamSynchronize(st), "stream sync");
+  if (!n_bytes) return WARPJQ_OK;
+  if (!c.overflow && (c.first_bad != ~0ull || c.first_te_bad != ~0ull || c.fb_count) && c.fb_count <= NOALLOC_FB) {
+    hipLaunchKernelGGL(k_res_prefix, dim3(1), dim3(1024), 0, st, tile, ntiles);
+    hipLaunchKernelGGL(k_res_records, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (c.fb_count + 255) / 256))),
+                       dim3(256), 0, st, ctr, tile, (uint32_t *)(S + L.fb_idx), (const uint32_t *)(S + L.fb_tile),
+                       NOALLOC_FB);
+    HIP_TRY(hipGetLastError(), "k_res");
+    HIP_TRY(hipMemcpyAsync(&c, ctr, sizeof c, hipMemcpyDeviceToHost, st), "D2H(counters)");
+    HIP_TRY(hipStreamSynchronize(st), "stream sync");
+  }
+  AggOut ao;
+  HIP_TRY(hipMemcpyAsync(&ao, S + L.aggout, sizeof ao, hipMemcpyDeviceToHost, st), "D2H(agg)");
+  HIP_TRY(hipStreamSynchronize(st), "stream sync");
+  out->n_lines = c.n_lines;
+  out->n_blank = c.n_blank;
+  out->n_invalid = c.n_invalid;
+  out->n_type_error = c.n_type_error;
+  out->n_fallback = c.fb_count;  // counted; their offsets stay on the device
+  out->first_invalid = c.first_invalid;
+  out->chunk_overflow = c.overflow || c.fb_count > NOALLOC_FB;
+  if (out->chunk_overflow) {
+    out->n_lines = 0;
+    return WARPJQ_OK;
+  }
+  out->agg = ao.agg;
+  out->agg.sum = warpjq_amd_ext_value(&ao.ext);
+  out->agg.saw_non_numeric = c.saw_non_numeric;
+  if (ext) *ext = ao.ext;
   return WARPJQ_OK;
 }
 
