@@ -295,6 +295,23 @@ def bench_served(args, w, np):
         log("served", r)
     st = srv.stats()
     srv.close()
+    # plain per-request launches beside it: each request alone, one after
+    # another, through a fresh context with its dataset copied from host
+    # memory (warpjq_amd_run_buffer), as a host without the server would run it
+    plain_reqs = [(served_query(rng), int(rng.integers(len(ids)))) for _ in range(args.plain_requests)]
+    plain_lat = []
+    t_start = time.perf_counter()
+    for q, d in plain_reqs:
+        a, b = bounds[d]
+        t = time.perf_counter()
+        w.run(q, host[a:b], chunk_bytes=64 << 20)
+        plain_lat.append((time.perf_counter() - t) * 1e3)
+    plain_wall = time.perf_counter() - t_start
+    plain_bytes = sum(sizes_b[d] for _, d in plain_reqs)
+    plain = {"requests": len(plain_reqs), "p50_ms": round(float(np.percentile(plain_lat, 50)), 3),
+             "p99_ms": round(float(np.percentile(plain_lat, 99)), 3),
+             "req_per_s": round(len(plain_reqs) / plain_wall, 1), "gbps": round(plain_bytes / plain_wall / 1e9, 1)}
+    log("plain per-request", plain)
     best = max(results, key=lambda r: r["gbps"])
     line = {
         "metric": "input GB/s of a Poisson mix of 12 query templates over 64 resident nginx datasets (served form)",
@@ -315,6 +332,8 @@ def bench_served(args, w, np):
                    "evictions": st["evictions"], "engines_opened": st["engines_opened"],
                    "engines_reprogrammed": st["engines_reprogrammed"]},
         "rates": results,
+        "plain_per_request": plain,
+        "served_vs_plain": round(best["gbps"] / plain["gbps"], 2) if plain["gbps"] else None,
     }
     print(json.dumps(line), flush=True)
 
@@ -390,6 +409,7 @@ def main():
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--requests", type=int, default=600)
     ap.add_argument("--rates", type=float, nargs="+", default=[100.0, 500.0, 2000.0, 5000.0])
+    ap.add_argument("--plain-requests", type=int, default=60, help="served: requests run one at a time, unserved")
     ap.add_argument("--files", type=int, default=4096)
     ap.add_argument("--long-frac", type=float, default=0.01, help="files mix: share of files with a long line")
     ap.add_argument("--graphs", action="store_true", help="files mix: also the per-file form with hipGraph replay")
@@ -536,6 +556,27 @@ def main():
                          "kernel_ms": round(st["kernel_ms"], 1), "chunks": st["chunks"],
                          "h2d_gbps": round(n / (st["h2d_ms"] / 1e3) / 1e9, 1) if st["h2d_ms"] else None}
         log("e2e", extras["e2e"])
+        # the nested workload's group_by over 1M users on a 1 GiB sample
+        # (kernel chain; the full 4 GiB form is --workload nested)
+        nh, nn, nlines, nbad = w.gen_nested(1 << 30, seed=args.seed, threads=min(16, os.cpu_count() or 1))
+        nd = torch.from_numpy(nh).to("cuda")
+        torch.cuda.synchronize()
+        for q in ("group_by(.user) | sum(.bytes)", "count"):
+            c3 = w.Context(q, chunk_bytes=512 << 20, n_slots=1)
+            chain, groups = 0.0, 0
+            for rep in range(2):
+                chain, groups = 0.0, 0
+                for a, b in split_points(nh, nn, 1 << 30):
+                    c3.submit_device(0, nd.data_ptr() + a, b - a)
+                    r = c3.wait(0)
+                    if r.group_overflow:
+                        raise SystemExit(f"{q}: group table overflow")
+                    groups += r.n_groups
+                    chain += c3.timing(0)[1]
+            c3.close()
+            extras["nested_1g " + q] = {"kernel_gbps": round(nn / (chain / 1e3) / 1e9, 1), "groups": groups}
+            log("nested", q, extras["nested_1g " + q])
+        del nd
 
     # CPU baseline: the oracle (the reference CPU backend restated) on the
     # parity sample, at -j 8 (the reference's published 8-core figure,

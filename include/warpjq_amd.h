@@ -128,12 +128,25 @@ warpjq_status warpjq_amd_set_arena(warpjq_ctx *ctx, uint32_t arena_bytes);
 
 /* Autotuner: times the one-pass kernel on the first 1 GiB of the resident
  * bytes (the slot capacity for row outputs) under a dozen claim / arena
- * pairs, keeps the fastest for this context and returns it with its time.
+ * pairs (the bytes per ticket and the lines each wave stages at once), and
+ * for grouped contexts then under group-table sizes from 2^14 slots up
+ * (warpjq_amd_group_table reads the choice); keeps the fastest for this
+ * context and returns the claim / arena pair with its time.
  * Uses slot 0, which must be idle. Results do not depend on the choice. No
  * counterpart in the reference (its tunables are compile-time constants,
  * kernels.cu:52-89). */
 warpjq_status warpjq_amd_autotune(warpjq_ctx *ctx, const uint8_t *d_bytes, uint64_t n_bytes, uint32_t *claim_out,
                                   uint32_t *arena_out, float *best_ms_out, char *err, size_t err_len);
+
+/* Group table slots in use for a grouped context (a power of two, 4,096 up
+ * to what the context allocated, 2^16..2^22 by chunk size). Fewer slots keep
+ * probes of low-cardinality keys in fewer cache lines; a chunk with more
+ * groups than fit sets group_overflow and the host splits it, so results do
+ * not depend on it. warpjq_amd_autotune picks it for grouped contexts. No
+ * counterpart in the reference (its table is a fixed 65,536 slots,
+ * kernels.cu:57-60). */
+warpjq_status warpjq_amd_set_group_table(warpjq_ctx *ctx, uint32_t slots);
+uint32_t warpjq_amd_group_table(const warpjq_ctx *ctx);
 
 /* Keep up to `per_slot` captured kernel chains (hipGraph) per slot: a
  * resident submit (warpjq_amd_submit_device / _spans) whose span table

@@ -297,3 +297,33 @@ def test_autotune_keeps_results(tmp_path):
     finally:
         ctx.close()
         buf.free()
+
+
+def test_autotune_group_table_keeps_results(tmp_path):
+    # grouped contexts: the autotuner also picks the group table's slots in
+    # use; every size gives the same groups, and a table too small for the
+    # chunk's keys reports group_overflow rather than a wrong answer
+    data = corpus.nginx(52, 8000) + b"".join(b'{"host":"h%d","bytes":%d}\n' % (i, i) for i in range(3000))
+    buf = w.DeviceBuffer(data)
+    q = "group_by(.host) | sum(.bytes)"
+    ctx = w.Context(q, chunk_bytes=1 << 22)
+    try:
+        ctx.submit_device(0, buf.ptr, len(data))
+        r = ctx.wait(0)
+        def groups(r):
+            return sorted((data[g.key_off:g.key_off + g.key_len], g.count, g.sum)
+                          for g in (r.groups[i] for i in range(r.n_groups)))
+        want = groups(r)
+        claim, arena, ms = ctx.autotune(buf.ptr, len(data), w.TuneCache(str(tmp_path / "t.json")))
+        assert ctx.group_table >= 2 * len(want)
+        for slots in (4096, 16384, ctx.group_table):
+            ctx.set_group_table(slots)
+            ctx.submit_device(0, buf.ptr, len(data))
+            r = ctx.wait(0)
+            assert not r.group_overflow
+            assert groups(r) == want
+        with pytest.raises(ValueError):
+            ctx.set_group_table(3000)
+    finally:
+        ctx.close()
+        buf.free()

@@ -233,3 +233,21 @@ def test_conformance_documents_through_the_kernels():
         cpu, cst = oracle.run(".", b, threads=1)
         gpu, gst = w.run(".", b)
         assert (gpu, gst["malformed"]) == (cpu, cst["malformed"]), d
+
+
+def test_program_tables_past_16_kib():
+    # a query whose tables (here a long string literal and a long key) do not
+    # fit beside the default line arena: the tables are staged whole and the
+    # arena shrinks, the answer is the oracle's
+    lit = "x" * 30000
+    key = "k" * 9000
+    lines = []
+    for i in range(3000):
+        host = lit if i % 7 == 0 else "h%d" % (i % 5)
+        lines.append('{"host":"%s","%s":%d,"bytes":%d}' % (host, key, i, i))
+    data = ("\n".join(lines) + "\n").encode()
+    for q in ('select(.host == "%s") | count' % lit, 'select(.host != "%s") | sum(.bytes)' % lit,
+              'select(.host == "%s") | {b: .bytes, k: .%s}' % (lit, key), "group_by(.%s) | count" % key):
+        cpu, _ = oracle.run(q, data, threads=2)
+        gpu, gst = w.run(q, data, chunk_bytes=1 << 20)
+        assert gpu == cpu, q[:60]

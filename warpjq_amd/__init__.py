@@ -111,7 +111,8 @@ ADDITIVE_SYMBOLS = ["warpjq_amd_ext_value", "warpjq_amd_wait_ext", "warpjq_amd_s
                     "warpjq_amd_free", "warpjq_amd_compile", "warpjq_amd_program_view",
                     "warpjq_amd_program_free", "warpjq_amd_gen_nginx", "warpjq_amd_gen_nested",
                     "warpjq_amd_first_type_error", "warpjq_amd_noalloc_max_bytes",
-                    "warpjq_amd_aggregate_device_noalloc"]
+                    "warpjq_amd_aggregate_device_noalloc", "warpjq_amd_set_group_table",
+                    "warpjq_amd_group_table"]
 
 
 def build(jobs=8):
@@ -367,7 +368,9 @@ class Context:
                 else:
                     self.set_claim(hit[0])
                     self.set_arena(hit[1])
-                self._claim, self._arena = hit
+                self._claim, self._arena = hit[0], hit[1]
+                if len(hit) > 2 and hit[2]:
+                    self.set_group_table(hit[2])
                 return hit[0], hit[1], None
         c, a, ms = u32(), u32(), ctypes.c_float()
         e = _err()
@@ -376,8 +379,21 @@ class Context:
             raise DeviceError(f"autotune: {STATUS_NAMES.get(rc, rc)}: {e.value.decode()}")
         self._claim, self._arena = c.value, a.value
         if cache is not None:
-            cache.put(key, (c.value, a.value))
+            pv = self.prog.view.contents
+            grouped = pv.output_kind == OUT_AGG and pv.group_path != 0xFFFFFFFF
+            cache.put(key, (c.value, a.value, self.group_table if grouped else 0))
         return c.value, a.value, ms.value
+
+    @property
+    def group_table(self):
+        """Group-table slots in use (grouped contexts)."""
+        L = lib()
+        L.warpjq_amd_group_table.restype = u32
+        return L.warpjq_amd_group_table(self.ctx)
+
+    def set_group_table(self, slots):
+        if lib().warpjq_amd_set_group_table(self.ctx, slots) != 0:
+            raise ValueError("group table: a power of two from 4096 up to the context's allocation, grouped queries")
 
     def set_graphs(self, per_slot):
         """Cache up to `per_slot` captured chains (hipGraph) per slot for resident submits."""
@@ -473,7 +489,7 @@ class Context:
 
 class TuneCache:
     """Autotuner choices persisted as JSON: (device, kernel width, output
-    shape, line-length class) -> (claim, arena)."""
+    shape, line-length class) -> (claim, arena, group-table slots or 0)."""
 
     def __init__(self, path):
         import json

@@ -522,6 +522,7 @@ struct warpjq_ctx {
   // device copies of the program tables
   void *d_tables = nullptr;
   uint32_t tab_bytes = 0;
+  uint32_t tab_cap = 0;  // bytes allocated at d_tables
   DProg dprog{};
   Slot *slots = nullptr;
   uint32_t n_slots = 0;
@@ -529,7 +530,8 @@ struct warpjq_ctx {
   uint64_t max_lines = 0;
   uint32_t fb_cap = 0;
   uint64_t out_cap = 0;
-  uint32_t group_cap = 0;
+  uint32_t group_cap = 0;     // slots allocated per slot's table
+  uint32_t group_active = 0;  // slots in use (a power of two <= group_cap; the autotuner may lower it)
   uint32_t grid = 0;
   uint32_t max_tiles = 0;
   uint32_t claim = CLAIM_DEFAULT;
@@ -767,9 +769,30 @@ warpjq_status warpjq_abi_check(const uint64_t *sizes, size_t n) {
   return WARPJQ_OK;
 }
 
-// Uploads the program tables (one allocation of TAB_LDS_MAX bytes, kept for
-// the context's life) and points ctx->dprog at them; sets the LDS table
-// size and the grid that follows from it. ctx->host must hold the program.
+// LDS one workgroup may take on this device, and what the grouped main
+// kernel's static table adds to its dynamic LDS.
+static uint32_t lds_limit() {
+  static uint32_t lim = 0;
+  if (!lim) {
+    hipDeviceProp_t p;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    lim = hipGetDeviceProperties(&p, dev) == hipSuccess && p.sharedMemPerBlock ? (uint32_t)p.sharedMemPerBlock : 65536u;
+  }
+  return lim;
+}
+
+// Whether the main kernel's LDS for `arena` and `tab` bytes of tables fits a
+// workgroup.
+static bool lds_fits(const warpjq_ctx *ctx, uint32_t arena, uint32_t tab);
+
+// Uploads the program tables (kept for the context's life, grown when a
+// reprogrammed query needs more) and points ctx->dprog at them; sets the LDS
+// table size and the grid that follows from it. Tables are staged in LDS per
+// workgroup: up to 16 KiB beside the default arena; larger ones (a long
+// string literal, many paths) are staged whole with the arena shrunk until
+// both fit the workgroup's LDS, which lowers occupancy but not results.
+// ctx->host must hold the program.
 static warpjq_status put_program(warpjq_ctx *ctx, const warpjq_program *prog, char *err, size_t err_len) {
   std::vector<uint8_t> buf;
   const DProg &h = ctx->host.d;
@@ -783,11 +806,29 @@ static warpjq_status put_program(warpjq_ctx *ctx, const warpjq_program *prog, ch
   size_t o_po = put_table(buf, prog->prefix_off, prog->n_fields);
   size_t o_pl = put_table(buf, prog->prefix_len, prog->n_fields);
   buf.resize((buf.size() + 15) & ~(size_t)15);
-  if (buf.size() > TAB_LDS_MAX) {
-    set_err(err, err_len, "query is too large for the kernels", "its tables exceed the 16 KiB staged in LDS");
-    return WARPJQ_ERR_INVALID_ARG;
+  const uint32_t tb = (uint32_t)buf.size();
+  if (!lds_fits(ctx, ctx->arena, tb)) {
+    // shrink the arena (and then the claim) until tables and arena fit
+    uint32_t claim = ctx->claim, arena = ctx->arena;
+    while (!lds_fits(ctx, arena, tb) && arena > claim + TAIL_MAX) arena -= 256;
+    while (!lds_fits(ctx, arena, tb) && claim > CLAIM_MIN) {
+      claim -= STEP_BYTES;
+      arena = claim + TAIL_MAX;
+    }
+    if (!lds_fits(ctx, arena, tb)) {
+      set_err(err, err_len, "query is too large for the kernels",
+              "its tables and the smallest line arena do not fit one workgroup's LDS");
+      return WARPJQ_ERR_INVALID_ARG;
+    }
+    ctx->claim = claim;
+    ctx->arena = arena;
   }
-  if (!ctx->d_tables) HIP_TRY(hipMalloc(&ctx->d_tables, TAB_LDS_MAX), "uploading the compiled query");
+  if (ctx->tab_cap < std::max<uint32_t>(tb, TAB_LDS_MAX)) {
+    if (ctx->d_tables) HIP_TRY(hipFree(ctx->d_tables), "uploading the compiled query");
+    ctx->d_tables = nullptr;
+    ctx->tab_cap = std::max<uint32_t>(tb, TAB_LDS_MAX);
+    HIP_TRY(hipMalloc(&ctx->d_tables, ctx->tab_cap), "uploading the compiled query");
+  }
   if (!buf.empty())
     HIP_TRY(hipMemcpy(ctx->d_tables, buf.data(), buf.size(), hipMemcpyHostToDevice), "uploading the compiled query");
   ctx->tab_bytes = (uint32_t)buf.size();
@@ -805,6 +846,11 @@ static warpjq_status put_program(warpjq_ctx *ctx, const warpjq_program *prog, ch
   d.prefix_len = (const uint32_t *)(base + o_pl);
   ctx->dprog = d;
   return WARPJQ_OK;
+}
+
+static bool lds_fits(const warpjq_ctx *ctx, uint32_t arena, uint32_t tab) {
+  const uint32_t stat = ctx->grouped ? (uint32_t)sizeof(LGroup) : 0u;
+  return main_lds_bytes(arena, tab) + stat <= lds_limit();
 }
 
 warpjq_status warpjq_ctx_create(const warpjq_program *prog, uint64_t max_chunk_bytes, uint32_t n_slots,
@@ -838,6 +884,7 @@ warpjq_status warpjq_ctx_create(const warpjq_program *prog, uint64_t max_chunk_b
   uint64_t g = 65536;
   while (g < max_chunk_bytes / 128 && g < (1u << 22)) g <<= 1;
   ctx->group_cap = (uint32_t)g;
+  ctx->group_active = (uint32_t)g;
   // tiles for the largest chunk a slot or a resident buffer may hold
   ctx->max_tiles = (uint32_t)((MAX_CHUNK + 16 + CLAIM_MIN - 1) / CLAIM_MIN) + 1;
 
@@ -930,6 +977,7 @@ static warpjq_status enqueue_chain(warpjq_ctx *ctx, Slot &s, uint32_t ns, uint32
     a.tile_state = s.d_tile_state;
     a.parts = s.d_parts;
     a.gt = s.gt;
+    a.gt.cap = ctx->group_active;
     a.row_len = s.d_row_len;
     a.sel_start = s.d_sel_start;
     a.max_lines = ctx->rows ? ctx->max_lines : 0;
@@ -965,7 +1013,9 @@ static warpjq_status enqueue_chain(warpjq_ctx *ctx, Slot &s, uint32_t ns, uint32
     if (timed) HIP_TRY(hipEventRecord(s.evm1, st), "hipEventRecord");
     if (ctx->dprog.output_kind == WARPJQ_OUT_AGG) {
       if (ctx->grouped) {
-        hipLaunchKernelGGL(k_group_compact, dim3(1024), dim3(256), 0, st, s.gt, s.d_ctr, s.d_groups, s.d_gext, s.d_spans,
+        GroupTab gt = s.gt;
+        gt.cap = ctx->group_active;
+        hipLaunchKernelGGL(k_group_compact, dim3(1024), dim3(256), 0, st, gt, s.d_ctr, s.d_groups, s.d_gext, s.d_spans,
                            ns, s.keys_blob ? s.d_keyblob : nullptr, KEYBLOB_BYTES);
       } else {
         hipLaunchKernelGGL(k_agg_final, dim3(1), dim3(256), 0, st, s.d_parts, grid, s.d_ctr, s.d_aggout);
@@ -1334,7 +1384,8 @@ warpjq_status warpjq_amd_set_claim(warpjq_ctx *ctx, uint32_t claim_bytes) {
 }
 
 warpjq_status warpjq_amd_set_arena(warpjq_ctx *ctx, uint32_t arena_bytes) {
-  if (!ctx || arena_bytes % 256 || arena_bytes > ARENA_MAX || arena_bytes < ctx->claim + TAIL_MAX)
+  if (!ctx || arena_bytes % 256 || arena_bytes > ARENA_MAX || arena_bytes < ctx->claim + TAIL_MAX ||
+      !lds_fits(ctx, arena_bytes, ctx->tab_bytes))
     return WARPJQ_ERR_INVALID_ARG;
   ctx->arena = arena_bytes;
   ctx->grid = main_grid(ctx);
@@ -1357,7 +1408,7 @@ warpjq_status warpjq_amd_autotune(warpjq_ctx *ctx, const uint8_t *d_bytes, uint6
   uint32_t bc = claim0, ba = arena0;
   for (const auto &c : cand) {
     const uint32_t cl = c[0] << 10, ar = c[1] << 10;
-    if (cl + TAIL_MAX > ar || ar > ARENA_MAX) continue;
+    if (cl + TAIL_MAX > ar || ar > ARENA_MAX || !lds_fits(ctx, ar, ctx->tab_bytes)) continue;
     ctx->arena = ar;  // both set together: the order of the two setters would matter
     ctx->claim = cl;
     ctx->grid = main_grid(ctx);
@@ -1389,9 +1440,62 @@ warpjq_status warpjq_amd_autotune(warpjq_ctx *ctx, const uint8_t *d_bytes, uint6
   ctx->arena = ba;
   ctx->grid = main_grid(ctx);
   ctx->prog_gen++;
+  if (ctx->grouped) {
+    // the group table's slots in use: the smallest tables that still hold
+    // twice the sample's groups keep the probes in fewer cache lines; the
+    // fastest of those is kept (a chunk with more groups than it holds is
+    // split by the host, so the choice never changes a result)
+    uint32_t ng = 0;
+    {
+      warpjq_status st = warpjq_amd_submit_device(ctx, 0, d_bytes, sample, err, err_len);
+      warpjq_chunk_result r;
+      if (st == WARPJQ_OK) st = warpjq_wait(ctx, 0, &r, err, err_len);
+      if (st != WARPJQ_OK) return st;
+      ng = r.n_groups;
+    }
+    uint32_t bg = ctx->group_cap;
+    float gbest = 1e30f;
+    for (uint32_t g = 1u << 14; g <= ctx->group_cap; g <<= 2) {
+      if (g < 2ull * ng) continue;
+      ctx->group_active = g;
+      ctx->prog_gen++;
+      float t = 1e30f;
+      for (int rep = 0; rep < 3; rep++) {
+        warpjq_status st = warpjq_amd_submit_device(ctx, 0, d_bytes, sample, err, err_len);
+        warpjq_chunk_result r;
+        if (st == WARPJQ_OK) st = warpjq_wait(ctx, 0, &r, err, err_len);
+        if (st != WARPJQ_OK) {
+          ctx->group_active = ctx->group_cap;
+          return st;
+        }
+        float m = 0.f;
+        warpjq_amd_slot_timing(ctx, 0, nullptr, nullptr, &m);
+        if (rep && m < t && !r.group_overflow) t = m;
+      }
+      if (t < gbest) {
+        gbest = t;
+        bg = g;
+      }
+    }
+    ctx->group_active = bg;
+    ctx->prog_gen++;
+    if (gbest < best) best = gbest;
+  }
   if (claim_out) *claim_out = bc;
   if (arena_out) *arena_out = ba;
   if (best_ms_out) *best_ms_out = best;
+  return WARPJQ_OK;
+}
+
+uint32_t warpjq_amd_group_table(const warpjq_ctx *ctx) { return ctx ? ctx->group_active : 0; }
+
+warpjq_status warpjq_amd_set_group_table(warpjq_ctx *ctx, uint32_t slots) {
+  if (!ctx || !ctx->grouped || slots < 4096 || (slots & (slots - 1)) || slots > ctx->group_cap)
+    return WARPJQ_ERR_INVALID_ARG;
+  for (uint32_t i = 0; i < ctx->n_slots; i++)
+    if (hipStreamSynchronize(ctx->slots[i].stream) != hipSuccess) return WARPJQ_ERR_CUDA;
+  ctx->group_active = slots;
+  ctx->prog_gen++;
   return WARPJQ_OK;
 }
 
