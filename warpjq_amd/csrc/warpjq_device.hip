@@ -953,17 +953,34 @@ uint64_t warpjq_max_lines(const warpjq_ctx *ctx) { return ctx ? ctx->max_lines :
 
 }  // extern "C"
 
+// The span table's copy to the device and the fixed-size results' copy back,
+// on either side of the kernels (a captured graph holds kernel nodes only:
+// the copies are queued around its launch).
+static warpjq_status copy_spans_in(Slot &s, uint32_t ns, char *err, size_t err_len) {
+  HIP_TRY(hipMemcpyAsync(s.d_spans, s.h_spans, (size_t)ns * sizeof(Span), hipMemcpyHostToDevice, s.stream),
+          "H2D(spans)");
+  return WARPJQ_OK;
+}
+static warpjq_status copy_results_out(warpjq_ctx *ctx, Slot &s, char *err, size_t err_len) {
+  HIP_TRY(hipMemcpyAsync(s.h_ctr, s.d_ctr, sizeof(DevCtr), hipMemcpyDeviceToHost, s.stream), "D2H(counters)");
+  if (ctx->dprog.output_kind == WARPJQ_OUT_AGG && !ctx->grouped)
+    HIP_TRY(hipMemcpyAsync(s.h_aggout, s.d_aggout, sizeof(AggOut), hipMemcpyDeviceToHost, s.stream), "D2H(agg)");
+  return WARPJQ_OK;
+}
+
 // Queues the kernel chain over the span table already in s.h_spans[0..ns)
 // on the slot's stream. `timed`: hipEvents around the main kernel (left out
-// of a captured graph).
+// of a captured graph); `copies`: the span and result copies too (left out
+// of a captured graph, whose caller queues them around the launch).
 static warpjq_status enqueue_chain(warpjq_ctx *ctx, Slot &s, uint32_t ns, uint32_t ntiles, uint64_t total, bool timed,
-                                   char *err, size_t err_len) {
+                                   bool copies, char *err, size_t err_len) {
   hipStream_t st = s.stream;
   const uint32_t claim = ctx->claim;
+  if (copies && total > 0)
+    if (warpjq_status cs = copy_spans_in(s, ns, err, err_len)) return cs;
   hipLaunchKernelGGL(k_reset, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (ntiles + 255) / 256))), dim3(256),
                      0, st, s.d_ctr, s.d_tile_state, ntiles);
   if (total > 0) {
-    HIP_TRY(hipMemcpyAsync(s.d_spans, s.h_spans, (size_t)ns * sizeof(Span), hipMemcpyHostToDevice, st), "H2D(spans)");
     KArgs a{};
     a.spans = s.d_spans;
     a.n_spans = ns;
@@ -1042,9 +1059,7 @@ static warpjq_status enqueue_chain(warpjq_ctx *ctx, Slot &s, uint32_t ns, uint32
     }
     HIP_TRY(hipGetLastError(), "kernel chain");
   }
-  HIP_TRY(hipMemcpyAsync(s.h_ctr, s.d_ctr, sizeof(DevCtr), hipMemcpyDeviceToHost, st), "D2H(counters)");
-  if (ctx->dprog.output_kind == WARPJQ_OUT_AGG && !ctx->grouped)
-    HIP_TRY(hipMemcpyAsync(s.h_aggout, s.d_aggout, sizeof(AggOut), hipMemcpyDeviceToHost, st), "D2H(agg)");
+  if (copies) return copy_results_out(ctx, s, err, err_len);
   return WARPJQ_OK;
 }
 
@@ -1084,7 +1099,7 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const warpjq_amd_spa
         hit = &g;
     if (!hit) {
       HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-      warpjq_status cs = enqueue_chain(ctx, s, ns, ntiles, total, false, err, err_len);
+      warpjq_status cs = enqueue_chain(ctx, s, ns, ntiles, total, false, false, err, err_len);
       hipGraph_t graph = nullptr;
       hipError_t ce = hipStreamEndCapture(st, &graph);
       if (cs != WARPJQ_OK) {
@@ -1106,11 +1121,13 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const warpjq_amd_spa
       hit = &s.graphs.back();
     }
     hit->last = ++s.graph_tick;
+    if (warpjq_status cs = copy_spans_in(s, ns, err, err_len)) return cs;
     HIP_TRY(hipEventRecord(s.evm0, st), "hipEventRecord");
     HIP_TRY(hipGraphLaunch(hit->exec, st), "hipGraphLaunch");
     HIP_TRY(hipEventRecord(s.evm1, st), "hipEventRecord");
+    if (warpjq_status cs = copy_results_out(ctx, s, err, err_len)) return cs;
   } else {
-    warpjq_status es = enqueue_chain(ctx, s, ns, ntiles, total, true, err, err_len);
+    warpjq_status es = enqueue_chain(ctx, s, ns, ntiles, total, true, true, err, err_len);
     if (es != WARPJQ_OK) return es;
   }
   HIP_TRY(hipEventRecord(s.ev2, st), "hipEventRecord");
