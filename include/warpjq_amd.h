@@ -192,6 +192,18 @@ int warpjq_amd_run_files(const char *query, const uint8_t *const *files, const u
                          int format, int on_invalid, uint64_t chunk_bytes, uint64_t max_line_bytes, uint8_t **out,
                          uint64_t *out_len, warpjq_amd_stats *stats, char *err, size_t err_len);
 
+/* The same over files named by path, as the reference's CLI opens its inputs
+ * (Input::open / Input::stdin / Input::chain, chunk.rs:60-205; run.rs:67-83):
+ * regular files are mapped read-only, "-" (or n_paths == 0) reads stdin and
+ * pipes are read as a stream with the partial last line carried into the
+ * next read (stream_chunks, chunk.rs:292-387); empty files add nothing. All
+ * paths are one stream, as in warpjq_amd_run_files. Returns 3 with "could
+ * not open `path`: ..." when a path cannot be opened, else the codes of
+ * warpjq_amd_run_buffer. */
+int warpjq_amd_run_paths(const char *query, const char *const *paths, uint32_t n_paths, int format, int on_invalid,
+                         uint64_t chunk_bytes, uint64_t max_line_bytes, uint8_t **out, uint64_t *out_len,
+                         warpjq_amd_stats *stats, char *err, size_t err_len);
+
 void warpjq_amd_free(void *p);
 
 /* The served form. A server keeps datasets resident in HBM within

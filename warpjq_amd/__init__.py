@@ -112,7 +112,7 @@ ADDITIVE_SYMBOLS = ["warpjq_amd_ext_value", "warpjq_amd_wait_ext", "warpjq_amd_s
                     "warpjq_amd_program_free", "warpjq_amd_gen_nginx", "warpjq_amd_gen_nested",
                     "warpjq_amd_first_type_error", "warpjq_amd_noalloc_max_bytes",
                     "warpjq_amd_aggregate_device_noalloc", "warpjq_amd_set_group_table",
-                    "warpjq_amd_group_table"]
+                    "warpjq_amd_group_table", "warpjq_amd_run_paths"]
 
 
 def build(jobs=8):
@@ -167,6 +167,9 @@ def lib():
         L.warpjq_amd_run_files.argtypes = [cp, P(ctypes.c_void_p), P(u64), u32, ctypes.c_int, ctypes.c_int, u64,
                                            u64, P(P(ctypes.c_uint8)), P(u64), P(Stats), cp, sz]
         L.warpjq_amd_run_files.restype = ctypes.c_int
+        L.warpjq_amd_run_paths.argtypes = [cp, P(cp), u32, ctypes.c_int, ctypes.c_int, u64, u64,
+                                           P(P(ctypes.c_uint8)), P(u64), P(Stats), cp, sz]
+        L.warpjq_amd_run_paths.restype = ctypes.c_int
         L.warpjq_amd_submit_pieces.argtypes = [ctypes.c_void_p, u32, u64, ctypes.c_void_p, u32, cp, sz]
         L.warpjq_amd_ctx_reprogram.argtypes = [ctypes.c_void_p, P(Program), cp, sz]
         L.warpjq_amd_set_graphs.argtypes = [ctypes.c_void_p, u32]
@@ -277,6 +280,25 @@ def run_files(query, files, fmt=FMT_NDJSON, on_invalid=SKIP, chunk_bytes=GPU_CHU
                                 ctypes.byref(out_len), ctypes.byref(st), e, 2048)
     del keep
     return _finish(L, rc, out, out_len, st, e)
+
+
+def run_paths(query, paths, fmt=FMT_NDJSON, on_invalid=SKIP, chunk_bytes=GPU_CHUNK, max_line=MAX_LINE):
+    """Runs `query` over files named by path, read as one stream ("-" or an
+    empty list reads stdin), on the GPU. Returns (output bytes, stats)."""
+    L = lib()
+    enc = [p.encode() if isinstance(p, str) else p for p in paths]
+    arr = (ctypes.c_char_p * max(1, len(enc)))(*enc)
+    out = P(ctypes.c_uint8)()
+    out_len = u64()
+    st = Stats()
+    e = ctypes.create_string_buffer(2048)
+    q = query.encode() if isinstance(query, str) else query
+    rc = L.warpjq_amd_run_paths(q, arr, len(enc), fmt, on_invalid, chunk_bytes, max_line, ctypes.byref(out),
+                                ctypes.byref(out_len), ctypes.byref(st), e, 2048)
+    return _finish(L, rc, out, out_len, st, e)
+
+
+CLI_PATH = os.path.join(_HERE, "warpjq-amd")
 
 
 def run(query, data, fmt=FMT_NDJSON, on_invalid=SKIP, chunk_bytes=GPU_CHUNK, max_line=MAX_LINE, n=None):

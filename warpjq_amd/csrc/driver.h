@@ -28,25 +28,59 @@ struct Piece {
 };
 using Batch = std::vector<Piece>;
 
-// Batches over a list of inputs read as one stream, the way the reference
-// chains files (chunk.rs:46-57, :198-203): a chunk never crosses an input,
-// each input's BOM is dropped (chunk.rs:210-212) and its last line is
-// complete at its end. Whole small inputs share a batch; a large one is cut
-// into newline-aligned chunks (gpu/mod.rs:697-810). A line longer than the
-// chunk but within max_line comes alone, as a batch of one `big` piece
+// Where a run's batches come from. engine_run sets cap / max_lines from the
+// context before the first next(); the bytes of a batch stay valid until
+// the batch after the next one has been taken (a run keeps at most two
+// batches on the device and redoes an overflowed one before taking more).
+struct Source {
+  uint64_t cap = 0, max_lines = 0;
+  const uint8_t *const *dfiles = nullptr;  // device copies of the inputs, or null
+  virtual ~Source() = default;
+  // false at the end of the input, or on an error (err set)
+  virtual bool next(Batch &b, std::string &err) = 0;
+};
+
+// Batches over a list of in-memory inputs read as one stream, the way the
+// reference chains files (chunk.rs:46-57, :198-203): a chunk never crosses
+// an input, each input's BOM is dropped (chunk.rs:210-212) and its last line
+// is complete at its end. Whole small inputs share a batch; a large one is
+// cut into newline-aligned chunks (gpu/mod.rs:697-810). A line longer than
+// the chunk but within max_line comes alone, as a batch of one `big` piece
 // (gpu/mod.rs:411-458, :824-857).
-struct Batcher {
+struct Batcher : Source {
   const uint8_t *const *files;
   const uint64_t *lens;
   uint32_t n_files;
-  uint64_t cap, max_lines, max_line;
+  uint64_t max_line;
   uint32_t max_pieces;
-  const uint8_t *const *dfiles = nullptr;  // device copies of `files`, or null
   uint32_t fi = 0;
   uint64_t pos = 0;
   bool started = false;
 
-  bool next(Batch &b, std::string &err);
+  Batcher(const uint8_t *const *files_, const uint64_t *lens_, uint32_t n_files_, uint64_t max_line_,
+          uint32_t max_pieces_)
+      : files(files_), lens(lens_), n_files(n_files_), max_line(max_line_), max_pieces(max_pieces_) {}
+  bool next(Batch &b, std::string &err) override;
+};
+
+// Batches over files named by path (Input::open / Input::stdin / Input::
+// chain, chunk.rs:60-205): regular files are mapped read-only and batched as
+// above; "-", pipes and other unmappable inputs are read as a stream, the
+// partial last line carried into the next read (chunk.rs:292-387). One
+// stream over the whole list: chunks never cross an input.
+struct PathSource : Source {
+  explicit PathSource(uint64_t max_line);
+  ~PathSource() override;
+  // Opens every path (nullptr or "-" = stdin). False with err set when one
+  // cannot be opened or mapped; nothing is read yet.
+  bool open(const char *const *paths, uint32_t n, std::string &err);
+  bool next(Batch &b, std::string &err) override;
+
+ private:
+  struct Part;
+  std::vector<Part *> parts_;
+  size_t cur_ = 0;
+  uint64_t max_line_;
 };
 
 // A lowered query with its device context, reusable run after run (one
@@ -68,7 +102,7 @@ int engine_open(Engine &g, const char *query, int format, uint64_t chunk_bytes, 
 int engine_reprogram(Engine &g, const char *query, int format, char *err, size_t err_len);
 void engine_close(Engine &g);
 // Runs every batch `src` yields; the return codes of warpjq_amd_run_buffer.
-int engine_run(Engine &g, Batcher &src, int format, int on_invalid, uint8_t **out, uint64_t *out_len,
+int engine_run(Engine &g, Source &src, int format, int on_invalid, uint8_t **out, uint64_t *out_len,
                warpjq_amd_stats *stats, char *err, size_t err_len);
 
 }  // namespace wjqd

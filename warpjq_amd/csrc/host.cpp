@@ -29,6 +29,11 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <cerrno>
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <string>
 #include <vector>
 
@@ -614,6 +619,205 @@ bool Batcher::next(Batch &b, std::string &err) {
 }
 
 
+// ---------------------------------------------------------------------------
+// Inputs named by path (chunk.rs:60-205, :292-387; gpu/mod.rs:662-914)
+// ---------------------------------------------------------------------------
+
+struct PathSource::Part {
+  // regular files, mapped read-only and batched in memory
+  std::vector<const uint8_t *> ptrs;
+  std::vector<uint64_t> lens;
+  std::vector<std::pair<void *, size_t>> maps;
+  Batcher *bat = nullptr;
+  // a stream (stdin, a pipe): three rotating buffers, so the two batches a
+  // run may still hold stay valid while the next buffer is read
+  bool stream = false;
+  int fd = -1;
+  bool own_fd = false;
+  std::vector<uint8_t> buf[3];
+  int cur = -1;
+  uint64_t pos = 0, end = 0;
+  std::vector<uint8_t> carry;
+  bool eof = false, first = true;
+
+  ~Part() {
+    delete bat;
+    for (auto &m : maps) munmap(m.first, m.second);
+    if (own_fd && fd >= 0) close(fd);
+  }
+
+  // Fills buf[cur] with carried bytes plus a read of up to `cap` more, cut
+  // after its last newline (stream_chunks, chunk.rs:292-387): the rest is
+  // carried. A buffer with no newline keeps growing until one arrives or the
+  // input ends, within max_line.
+  bool fill(uint64_t cap, uint64_t max_line, std::string &err) {
+    cur = (cur + 1) % 3;
+    std::vector<uint8_t> &B = buf[cur];
+    B.assign(carry.begin(), carry.end());
+    carry.clear();
+    pos = end = 0;
+    for (;;) {
+      const uint64_t base = B.size();
+      const uint64_t want = base >= cap ? cap : cap - base;
+      B.resize(base + want);
+      uint64_t filled = base;
+      while (filled < B.size()) {
+        const ssize_t k = read(fd, B.data() + filled, B.size() - filled);
+        if (k < 0) {
+          if (errno == EINTR) continue;
+          err = std::string("reading the input: ") + strerror(errno);
+          return false;
+        }
+        if (k == 0) {
+          eof = true;
+          break;
+        }
+        filled += (uint64_t)k;
+      }
+      B.resize(filled);
+      if (first && B.size() >= 3) {  // strip_bom, chunk.rs:210-212
+        first = false;
+        if (B[0] == 0xEF && B[1] == 0xBB && B[2] == 0xBF) B.erase(B.begin(), B.begin() + 3);
+      } else if (first && eof) {
+        first = false;
+      }
+      if (eof) {
+        end = B.size();
+        return true;
+      }
+      const uint8_t *nl = (const uint8_t *)memrchr(B.data(), '\n', B.size());
+      if (!nl) {
+        if (B.size() > max_line) {
+          err = "a line exceeds the " + std::to_string(max_line) + " byte limit (raise it with --max-line-bytes)";
+          return false;
+        }
+        continue;  // a line longer than what was read: read more
+      }
+      end = (uint64_t)(nl - B.data()) + 1;
+      carry.assign(B.begin() + (ptrdiff_t)end, B.end());
+      B.resize(end);
+      return true;
+    }
+  }
+
+  bool next_stream(Batch &b, std::string &err, uint64_t cap, uint64_t max_lines, uint64_t max_line) {
+    for (;;) {
+      if (cur >= 0 && pos < end) {
+        const uint8_t *d = buf[cur].data();
+        uint64_t p = pos;
+        Chunk c;
+        std::string e;
+        bool too_long = false;
+        if (!next_chunk(d, end, p, cap, max_lines, max_line, c, e, &too_long)) {
+          if (!too_long) {
+            err = e;
+            return false;
+          }
+          // one line longer than the chunk: alone (gpu/mod.rs:411-458)
+          const uint8_t *nl = (const uint8_t *)memchr(d + pos, '\n', end - pos);
+          const uint64_t le = nl ? (uint64_t)(nl - d) + 1 : end;
+          if (le - pos - (nl ? 1 : 0) > max_line) {
+            err = "a line exceeds the " + std::to_string(max_line) + " byte limit (raise it with --max-line-bytes)";
+            return false;
+          }
+          Piece pc{d + pos, le - pos};
+          pc.big = true;
+          b.push_back(pc);
+          pos = le;
+          return true;
+        }
+        b.push_back({d + c.off, c.len});
+        pos = p;
+        return true;
+      }
+      if (eof) return false;
+      if (!fill(cap, max_line, err)) return false;
+    }
+  }
+};
+
+PathSource::PathSource(uint64_t max_line) : max_line_(max_line) {}
+
+PathSource::~PathSource() {
+  for (Part *p : parts_) delete p;
+}
+
+bool PathSource::open(const char *const *paths, uint32_t n, std::string &err) {
+  auto stream_part = [&](int fd, bool own) {
+    Part *p = new Part;
+    p->stream = true;
+    p->fd = fd;
+    p->own_fd = own;
+    parts_.push_back(p);
+  };
+  if (n == 0) {  // no files: stdin (run.rs:67-76)
+    stream_part(0, false);
+    return true;
+  }
+  for (uint32_t i = 0; i < n; i++) {
+    const char *path = paths[i];
+    if (!path || !strcmp(path, "-")) {
+      stream_part(0, false);
+      continue;
+    }
+    const int fd = ::open(path, O_RDONLY | O_CLOEXEC);
+    if (fd < 0) {
+      err = std::string("could not open `") + path + "`: " + strerror(errno);
+      return false;
+    }
+    struct stat st;
+    if (fstat(fd, &st) != 0) {
+      err = std::string("could not open `") + path + "`: " + strerror(errno);
+      close(fd);
+      return false;
+    }
+    // only regular files are mapped (chunk.rs:61-80); an empty one adds nothing
+    if (S_ISREG(st.st_mode)) {
+      if (st.st_size == 0) {
+        close(fd);
+        continue;
+      }
+      void *m = mmap(nullptr, (size_t)st.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+      const int e = errno;
+      close(fd);
+      if (m == MAP_FAILED) {
+        err = std::string("could not map `") + path + "`: " + strerror(e);
+        return false;
+      }
+      madvise(m, (size_t)st.st_size, MADV_SEQUENTIAL);
+      madvise(m, (size_t)st.st_size, MADV_WILLNEED);
+      if (parts_.empty() || parts_.back()->stream) parts_.push_back(new Part);
+      Part *p = parts_.back();
+      p->maps.push_back({m, (size_t)st.st_size});
+      p->ptrs.push_back((const uint8_t *)m);
+      p->lens.push_back((uint64_t)st.st_size);
+      continue;
+    }
+    stream_part(fd, true);
+  }
+  return true;
+}
+
+bool PathSource::next(Batch &b, std::string &err) {
+  b.clear();
+  while (cur_ < parts_.size()) {
+    Part *p = parts_[cur_];
+    bool got;
+    if (p->stream) {
+      got = p->next_stream(b, err, cap, max_lines, max_line_);
+    } else {
+      if (!p->bat) p->bat = new Batcher(p->ptrs.data(), p->lens.data(), (uint32_t)p->ptrs.size(), max_line_, 4096);
+      p->bat->cap = cap;
+      p->bat->max_lines = max_lines;
+      got = p->bat->next(b, err);
+    }
+    if (got) return true;
+    if (!err.empty()) return false;
+    cur_++;
+  }
+  return false;
+}
+
 }  // namespace wjqd
 
 struct warpjq_amd_program {
@@ -701,7 +905,7 @@ void engine_close(Engine &g) {
   g.ctx = nullptr;
 }
 
-int engine_run(Engine &g, Batcher &src, int format, int on_invalid, uint8_t **out, uint64_t *out_len,
+int engine_run(Engine &g, Source &src, int format, int on_invalid, uint8_t **out, uint64_t *out_len,
                warpjq_amd_stats *stats, char *err, size_t err_len) {
   *out = nullptr;
   *out_len = 0;
@@ -927,7 +1131,7 @@ int engine_run(Engine &g, Batcher &src, int format, int on_invalid, uint8_t **ou
 
 extern "C" {
 
-static int run_once(const char *query, wjqd::Batcher &src, int format, int on_invalid, uint64_t chunk_bytes,
+static int run_once(const char *query, wjqd::Source &src, int format, int on_invalid, uint64_t chunk_bytes,
                     uint8_t **out, uint64_t *out_len, warpjq_amd_stats *stats, char *err, size_t err_len) {
   *out = nullptr;
   *out_len = 0;
@@ -943,7 +1147,7 @@ static int run_once(const char *query, wjqd::Batcher &src, int format, int on_in
 int warpjq_amd_run_buffer(const char *query, const uint8_t *data, uint64_t n, int format, int on_invalid,
                           uint64_t chunk_bytes, uint64_t max_line_bytes, uint8_t **out, uint64_t *out_len,
                           warpjq_amd_stats *stats, char *err, size_t err_len) {
-  wjqd::Batcher src{&data, &n, 1, 0, 0, max_line_bytes, 1};
+  wjqd::Batcher src(&data, &n, 1, max_line_bytes, 1);
   return run_once(query, src, format, on_invalid, chunk_bytes, out, out_len, stats, err, err_len);
 }
 
@@ -956,7 +1160,26 @@ int warpjq_amd_run_files(const char *query, const uint8_t *const *files, const u
     snprintf(err, err_len, "run_files: null file list");
     return 3;
   }
-  wjqd::Batcher src{files, lens, n_files, 0, 0, max_line_bytes, 4096};
+  wjqd::Batcher src(files, lens, n_files, max_line_bytes, 4096);
+  return run_once(query, src, format, on_invalid, chunk_bytes, out, out_len, stats, err, err_len);
+}
+
+int warpjq_amd_run_paths(const char *query, const char *const *paths, uint32_t n_paths, int format, int on_invalid,
+                         uint64_t chunk_bytes, uint64_t max_line_bytes, uint8_t **out, uint64_t *out_len,
+                         warpjq_amd_stats *stats, char *err, size_t err_len) {
+  *out = nullptr;
+  *out_len = 0;
+  memset(stats, 0, sizeof *stats);
+  if (n_paths && !paths) {
+    snprintf(err, err_len, "run_paths: null path list");
+    return 3;
+  }
+  wjqd::PathSource src(max_line_bytes);
+  std::string e;
+  if (!src.open(paths, n_paths, e)) {
+    snprintf(err, err_len, "%s", e.c_str());
+    return 3;
+  }
   return run_once(query, src, format, on_invalid, chunk_bytes, out, out_len, stats, err, err_len);
 }
 

@@ -287,7 +287,7 @@ void worker(warpjq_amd_server *s) {
       continue;
     }
     // no per-request scan of the bytes: line lengths were checked above
-    wjqd::Batcher src{hp.data(), lens.data(), (uint32_t)hp.size(), 0, 0, ~0ull, 65536};
+    wjqd::Batcher src(hp.data(), lens.data(), (uint32_t)hp.size(), ~0ull, 65536);
     src.dfiles = dp.data();
     r->rc = wjqd::engine_run(it->eng, src, r->fmt, r->oi, &r->out, &r->out_len, &r->st, err, sizeof err);
     if (r->rc) r->err = err;
