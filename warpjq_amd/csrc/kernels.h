@@ -76,6 +76,12 @@ using namespace wjqd;
 #define TAIL_PIECE 256u              // bytes of the next claim fetched per try
 #define TAIL_MAX 1024u               // a claim's last line may end this far past it
 #define ARENA_DEFAULT 18432u         // per-wave LDS arena (bytes)
+// grouped queries: the per-wave LDS group table (about 4 KiB) would leave
+// room for 7 workgroups per CU with the default arena; 6 KiB claims in a
+// 14 KiB arena keep 8, with room to prefetch the next claim (measured on the
+// nginx group_by(.host): 1.30 vs 1.58 ms per GiB)
+#define GROUP_CLAIM_DEFAULT 6144u
+#define GROUP_ARENA_DEFAULT 14336u
 #define ARENA_MAX 32768u             // offsets fit 15 bits; bit 15 marks unstaged lines
 #define QCAP 128u                    // per-wave LDS queue of line starts
 #define QMASK (QCAP - 1u)

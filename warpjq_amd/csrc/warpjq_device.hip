@@ -881,6 +881,10 @@ warpjq_status warpjq_ctx_create(const warpjq_program *prog, uint64_t max_chunk_b
   ctx->out_cap = max_chunk_bytes + max_chunk_bytes / 2 + (1u << 20);
   ctx->grouped = prog->output_kind == WARPJQ_OUT_AGG && prog->group_path != 0xFFFFFFFFu;
   ctx->rows = prog->output_kind != WARPJQ_OUT_AGG;
+  if (ctx->grouped) {
+    ctx->claim = GROUP_CLAIM_DEFAULT;
+    ctx->arena = GROUP_ARENA_DEFAULT;
+  }
   uint64_t g = 65536;
   while (g < max_chunk_bytes / 128 && g < (1u << 22)) g <<= 1;
   ctx->group_cap = (uint32_t)g;
