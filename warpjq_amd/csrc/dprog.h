@@ -133,7 +133,7 @@ inline bool build_dprog(const warpjq_program &p, DProgHost &h, std::string &err)
       err = "path steps out of range";
       return false;
     }
-    SlotPath sp{pp.step_count, pp.step_off, 0, 0};
+    SlotPath sp{pp.step_count, pp.step_off, 0, ~0u, {0, 0, 0, 0}};
     bool ok = pp.step_count <= WJQ_FUSED_STEPS;
     for (uint32_t k = 0; k < pp.step_count; k++) {
       const warpjq_step &st = p.steps[pp.step_off + k];
@@ -222,6 +222,24 @@ inline bool build_dprog(const warpjq_program &p, DProgHost &h, std::string &err)
     else lens0 |= 1u << 31;
   }
   d.key_lens0 = lens0;
+  // keys a minified flat line can hold verbatim, compared as words
+  uint32_t plain = 0;
+  for (uint32_t q = 0; q < p.n_needed; q++) {
+    if (!((fused >> q) & 1)) continue;
+    const warpjq_step &st = p.steps[h.spath[q].step_off];
+    if (st.kind != WARPJQ_STEP_KEY || st.key_len > 16) continue;
+    bool ok = true;
+    for (uint32_t k = 0; k < st.key_len; k++) {
+      const uint8_t c = p.blob[st.key_off + k];
+      if (c == '"' || c == '\\' || c < 0x20) ok = false;
+    }
+    if (!ok) continue;
+    SlotPath &sp = h.spath[q];
+    sp.klen = st.key_len;
+    for (uint32_t k = 0; k < st.key_len; k++) sp.kw[k >> 2] |= (uint32_t)p.blob[st.key_off + k] << (8 * (k & 3));
+    plain |= 1u << q;
+  }
+  d.plain_keys = plain;
   d.filter_mask = filter_mask;
   d.post_mask = all & ~filter_mask;
   d.has_filter = p.has_filter;

@@ -61,7 +61,9 @@ struct SlotPath {
   uint32_t nsteps;
   uint32_t step_off;  // into steps[]
   uint32_t fused;     // 1 when nsteps <= WJQ_FUSED_STEPS and indices < 0xFFFF
-  uint32_t _pad;
+  uint32_t klen;      // fused slot whose first step is a key of at most 16 bytes
+                      // with no '"', backslash or control byte: its length; else ~0
+  uint32_t kw[4];     // that key's bytes, little-endian words, zero past klen
 };
 
 // The device view of a lowered program (warpjq_program plus derived masks).
@@ -79,6 +81,7 @@ struct DProg {
   uint32_t n_cond;
   uint32_t fused_mask;   // slots captured by the walk
   uint32_t key_lens0;    // bit n: a fused slot's first step is a key of n bytes (bit 31: 31+ bytes, or no key)
+  uint32_t plain_keys;   // slots with SlotPath::klen set
   uint32_t filter_mask;  // slots the filter reads
   uint32_t post_mask;    // every other needed slot
   uint32_t has_filter;
@@ -140,6 +143,15 @@ WJQ_HD uint32_t non_digit(uint32_t x) {
 
 WJQ_HD int ctz32(uint32_t x) { return __builtin_ctz(x); }
 WJQ_HD int ctz64(uint64_t x) { return __builtin_ctzll(x); }
+
+// Bytes sh..sh+3 of the 8 bytes hi:lo, little-endian (sh < 4).
+WJQ_HD uint32_t funnel_bytes(uint32_t hi, uint32_t lo, uint32_t sh) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_alignbyte(hi, lo, sh);
+#else
+  return sh ? (lo >> (8 * sh)) | (hi << (32 - 8 * sh)) : lo;
+#endif
+}
 WJQ_HD int clz64(uint64_t x) { return __builtin_clzll(x); }
 
 // ---------------------------------------------------------------------------
@@ -718,9 +730,52 @@ struct NumVal {
   bool is_int;   // integer token with |value| <= 2^53
 };
 
+template <class R, class = void>
+struct has_dword_reads : std::false_type {};
+template <class R>
+struct has_dword_reads<R, std::void_t<decltype(&R::dword)>> : std::true_type {};
+
+// Value of the 4 digits in bytes 0-3 of w (byte 0 the most significant).
+WJQ_HD uint32_t digits4(uint32_t w) {
+  const uint32_t p = (w & 0x00FF00FFu) * 10u + ((w >> 8) & 0x00FF00FFu);
+  return (p & 0xFFFFu) * 100u + (p >> 16);
+}
+
 // Token [a, b) is a validated JSON number. Returns false when undecided.
 template <class R>
 WJQ_HD bool parse_number(R &rd, uint32_t a, uint32_t b, NumVal &out) {
+  if constexpr (has_dword_reads<R>::value && !wide_reader<R>::value) {
+    // 1 to 8 bytes, all digits: the integer from two words (the value a
+    // digit loop would give: leading zeros add nothing, below 2^53)
+    const uint32_t n = b - a;
+    if (n - 1u < 8u) {
+      const uint32_t k0 = a >> 2, sh = a & 3u;
+      const uint32_t d0 = rd.dword(k0), d1 = rd.dword(k0 + 1u), d2 = rd.dword(k0 + 2u);
+      const uint32_t mlo = n >= 4u ? ~0u : (1u << (8u * n)) - 1u;
+      const uint32_t mhi = n <= 4u ? 0u : n == 8u ? ~0u : (1u << (8u * (n - 4u))) - 1u;
+      const uint32_t lo = funnel_bytes(d1, d0, sh) & mlo, hi = funnel_bytes(d2, d1, sh) & mhi;
+      const uint32_t vlo = lo ^ (0x30303030u & mlo), vhi = hi ^ (0x30303030u & mhi);
+      const uint32_t big = ((((vlo & 0x7F7F7F7Fu) + 0x76767676u) | vlo) | (((vhi & 0x7F7F7F7Fu) + 0x76767676u) | vhi)) &
+                           0x80808080u;
+      if (!big) {
+        // right-align to 8 digits: shift left by 8 - n bytes
+        const uint32_t s8 = 8u * (8u - n);
+        uint32_t l2, h2;
+        if (s8 >= 32u) {
+          h2 = vlo << (s8 - 32u);
+          l2 = 0;
+        } else {
+          h2 = s8 ? (vhi << s8) | (vlo >> (32u - s8)) : vhi;
+          l2 = vlo << s8;
+        }
+        const uint32_t v = digits4(l2) * 10000u + digits4(h2);
+        out.v = (double)v;
+        out.iv = (int64_t)v;
+        out.is_int = true;
+        return true;
+      }
+    }
+  }
   uint32_t i = a;
   bool neg = rd.byte(i) == '-';
   if (neg) i++;
@@ -1613,6 +1668,32 @@ WJQ_HD uint32_t flat_key_hits(R &rd, const PP &P, uint32_t want, uint32_t ka, ui
   return hit;
 }
 
+// Slots among `cand` (plain keys of L <= 16 bytes, SlotPath::kw) equal to
+// the L bytes at x, which are read as aligned words.
+template <int NS, class R, class PP>
+WJQ_HD uint32_t flat_key_eq(R &rd, const PP &P, uint32_t cand, uint32_t x, uint32_t L) {
+  const uint32_t nw = (L + 3u) >> 2;
+  const uint32_t k0 = x >> 2, sh = x & 3u;
+  uint32_t d[5];
+#pragma unroll
+  for (uint32_t i = 0; i < 5; i++) d[i] = i <= nw ? rd.dword(k0 + i) : 0u;
+  uint32_t lw[4];
+#pragma unroll
+  for (uint32_t i = 0; i < 4; i++) {
+    lw[i] = i < nw ? funnel_bytes(d[i + 1], d[i], sh) : 0u;
+    if (i + 1u == nw && (L & 3u)) lw[i] &= (1u << (8u * (L & 3u))) - 1u;
+  }
+  uint32_t hit = 0;
+  while (cand) {
+    const uint32_t q = (uint32_t)ctz32(cand);
+    cand &= cand - 1;
+    const auto &sp = P.spath[q];
+    const uint32_t diff = (lw[0] ^ sp.kw[0]) | (lw[1] ^ sp.kw[1]) | (lw[2] ^ sp.kw[2]) | (lw[3] ^ sp.kw[3]);
+    if (!diff) hit |= 1u << q;
+  }
+  return hit;
+}
+
 // `i` is the line's '{'. True: the line is a valid flat object; the fused
 // slots, `te` and `end` are set as walk() sets them. False: nothing is
 // written; walk the line.
@@ -1645,7 +1726,33 @@ WJQ_HD bool flat_walk(R &rd, uint32_t i, const PP &P, SlotV (&sl)[NS], uint32_t 
     }
   }
   if (want) {
-    if (!(P.key_lens0 >> 31)) {
+    if (!(P.key_lens0 >> 31) && !(want & ~P.plain_keys)) {
+      // every wanted key is plain and at most 16 bytes: per key length L,
+      // key starts whose key end lies L + 1 bytes on, compared as words. A
+      // plain key holds no '"', so equal bytes mean that end is the key's
+      // own closing quote. Line order within a length (a duplicate key's
+      // later value wins); one slot has one length.
+      uint32_t lens = P.key_lens0;
+      while (lens) {
+        const uint32_t L = (uint32_t)ctz32(lens);
+        lens &= lens - 1;
+        uint32_t wl = 0;
+        for (uint32_t A = want; A; A &= A - 1) {
+          const uint32_t q = (uint32_t)ctz32(A);
+          if (P.spath[q].klen == L) wl |= 1u << q;
+        }
+#pragma unroll
+        for (int w = 0; w < FLAT_WORDS; w++) {
+          uint64_t cb = KSw[w] & flat_shr(KEw, w, L + 1u);
+          while (cb) {
+            const uint32_t kq = 64u * (uint32_t)w + (uint32_t)ctz64(cb);
+            cb &= cb - 1;
+            const uint32_t hit = flat_key_eq<NS>(rd, P, wl, B + kq + 1u, L);
+            if (hit) flat_take<NS>(rd, P, B, kq + L + 1u, hit, EDw, sl, t_err);
+          }
+        }
+      }
+    } else if (!(P.key_lens0 >> 31)) {
       // every wanted key is shorter than 31 bytes: candidates by length,
       // key starts whose key end lies len + 1 bytes on (KS & KE >> (len + 1)),
       // in line order (a duplicate key's later value wins)

@@ -371,7 +371,7 @@ struct LProg {
   const LDS_AS uint32_t *field_slot;
   const LDS_AS uint32_t *prefix_off;
   const LDS_AS uint32_t *prefix_len;
-  uint32_t n_slots, n_cond, fused_mask, key_lens0, filter_mask, post_mask, has_filter, output_kind, output_slot,
+  uint32_t n_slots, n_cond, fused_mask, key_lens0, plain_keys, filter_mask, post_mask, has_filter, output_kind, output_slot,
       agg_kind,
       agg_slot, group_slot, n_fields, suffix_off, suffix_len, csv_mode;
 };
@@ -398,6 +398,7 @@ __device__ __forceinline__ LProg stage_prog(const DProg &P, const uint8_t *tab, 
   L.n_cond = P.n_cond;
   L.fused_mask = P.fused_mask;
   L.key_lens0 = P.key_lens0;
+  L.plain_keys = P.plain_keys;
   L.filter_mask = P.filter_mask;
   L.post_mask = P.post_mask;
   L.has_filter = P.has_filter;
@@ -974,49 +975,59 @@ __device__ __forceinline__ uint32_t enqueue_lines(const KArgs &a, const LDS_AS u
   const uint32_t p0 = c.t0 + lane * S;
   const LDS_AS uint8_t *base = ar + (c.at - c.t0);  // base[q] is the staged byte at q position q
   uint32_t prev = lane == 0 ? c.carry0 : (base[p0 - 1] == '\n' ? 1u : 0u);
-  uint32_t st[8];
-  uint32_t cnt = 0, hi = 0;
+  // bit b of w[j]: a line starts at q position p0 + 32 j + b (a newline before it)
+  uint32_t w[4];
 #pragma unroll
   for (uint32_t k = 0; k < 8; k++) {
-    st[k] = 0;
+    uint32_t x = 0;
     if (k < nb) {
-      const uint32_t p = p0 + 16u * k;
-      const uint32_t m = nl_mask16(lds_ld16(base + p));
-      uint32_t x = ((m << 1) | prev) & 0xFFFFu;
+      const uint32_t m = nl_mask16(lds_ld16(base + p0 + 16u * k));
+      x = ((m << 1) | prev) & 0xFFFFu;
       prev = m >> 15;
-      // a span's first line starts at its first byte; nothing before it does
-      if (p + 16u <= c.start) x = 0;
-      else if (p <= c.start) x = (x & (0xFFFFu << (c.start - p))) | (1u << (c.start - p));
-      // keep starts in [c.resume, c.end)
-      if (p + 16u <= c.resume) x = 0;
-      else if (p < c.resume) x &= 0xFFFFu << (c.resume - p);
-      if (p >= c.end) x = 0;
-      else if (p + 16u > c.end) x &= (1u << (c.end - p)) - 1u;
-      st[k] = x;
-      cnt += __popc(x);
-      if (x) hi = p + 31u - (uint32_t)__builtin_clz(x);
     }
+    if (k & 1u) w[k >> 1] |= x << 16;
+    else w[k >> 1] = x;
+  }
+  // a span's first line starts at its first byte and nothing before it does;
+  // keep the starts in [max(start, resume), end)
+  const uint32_t lo = max(c.start, c.resume);
+  uint32_t cnt = 0, hi = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 4; j++) {
+    const uint32_t pj = p0 + 32u * j;
+    uint32_t x = w[j];
+    if (c.start - p0 < S && c.start - pj < 32u) x |= 1u << (c.start - pj);
+    if (lo > pj) x = lo - pj >= 32u ? 0u : x & (~0u << (lo - pj));
+    if (c.end <= pj) x = 0;
+    else if (c.end - pj < 32u) x &= (1u << (c.end - pj)) - 1u;
+    w[j] = x;
+    cnt += __popc(x);
+    if (x) hi = pj + 31u - (uint32_t)__builtin_clz(x);
   }
   uint32_t tot;
   const uint32_t ex = wave_excl_scan(cnt, tot);
   uint32_t next = c.end;  // the start ranked `room`, if the queue fills
   uint32_t r = ex;
-#pragma unroll
-  for (uint32_t k = 0; k < 8; k++) {
-    uint32_t bits = st[k];
-    while (bits) {
-      const uint32_t bpos = (uint32_t)__builtin_ctz(bits);
-      bits &= bits - 1;
-      const uint32_t q = p0 + 16u * k + bpos;
-      if (r < room) {
-        const uint32_t e = (qtail + r) & QMASK;
-        Q.qe[e] = (q - c.t0 + c.at) | (q == c.last_unstaged ? QE_UNSTAGED : 0u) | (c.slot << 16) |
-                  ((c.rank + r) << 19);
-      } else if (r == room) {
-        next = q;
-      }
-      r++;
+  // one pass per start of this lane (lines longer than the slice: at most
+  // one), lowest first, selected without branches
+  while (w[0] | w[1] | w[2] | w[3]) {
+    const bool s0 = w[0] != 0, s1 = !s0 && w[1] != 0, s2 = !s0 && !s1 && w[2] != 0;
+    const bool s3 = !s0 && !s1 && !s2;
+    const uint32_t x = s0 ? w[0] : s1 ? w[1] : s2 ? w[2] : w[3];
+    const uint32_t q = p0 + (s0 ? 0u : s1 ? 32u : s2 ? 64u : 96u) + (uint32_t)__builtin_ctz(x);
+    const uint32_t xl = x & (x - 1u);
+    w[0] = s0 ? xl : w[0];
+    w[1] = s1 ? xl : w[1];
+    w[2] = s2 ? xl : w[2];
+    w[3] = s3 ? xl : w[3];
+    if (r < room) {
+      const uint32_t e = (qtail + r) & QMASK;
+      Q.qe[e] = (q - c.t0 + c.at) | (q == c.last_unstaged ? QE_UNSTAGED : 0u) | (c.slot << 16) |
+                ((c.rank + r) << 19);
+    } else if (r == room) {
+      next = q;
     }
+    r++;
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) hi = max(hi, (uint32_t)__shfl_xor(hi, o, 64));
