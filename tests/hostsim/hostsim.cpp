@@ -21,6 +21,18 @@ struct HostReader {
     return v;
   }
   Blk16 block(uint32_t b) const { return Blk16{dword(4 * b), dword(4 * b + 1), dword(4 * b + 2), dword(4 * b + 3)}; }
+  // as the kernels' LDS reader: the flat walk starts at the line's dword
+  static constexpr uint32_t flat_align = 4u;
+  Blk16 blk(uint32_t pos) const { return Blk16{dword(pos / 4), dword(pos / 4 + 1), dword(pos / 4 + 2), dword(pos / 4 + 3)}; }
+};
+
+// The same bytes read only in aligned 16-byte blocks, as the kernels' HBM
+// reader does (the flat walk then starts at the line's 16-byte block).
+struct HostBlockReader {
+  HostReader r;
+  uint32_t byte(uint32_t i) const { return r.byte(i); }
+  uint32_t dword(uint32_t k) const { return r.dword(k); }
+  Blk16 block(uint32_t b) const { return r.block(b); }
 };
 
 struct StrSink {
@@ -224,8 +236,8 @@ int hs_validate_deep(const uint8_t *line, uint64_t n, uint32_t cap) {
 // type errors and end), 2 it accepted a line the walk decides otherwise.
 }  // extern "C"
 
-template <int NS>
-static int flat_vs_walk_t(Handle *h, HostReader &rd, uint32_t i) {
+template <int NS, class RD>
+static int flat_vs_walk_r(Handle *h, RD &rd, uint32_t i) {
   const DProg &P = h->dp.d;
   SlotV a[NS], b[NS];
   for (int q = 0; q < NS; q++) a[q] = b[q] = SlotV{0, 0, JK_MISSING};
@@ -239,6 +251,15 @@ static int flat_vs_walk_t(Handle *h, HostReader &rd, uint32_t i) {
     if (a[q].kind != JK_MISSING && (a[q].off != b[q].off || a[q].len != b[q].len)) return 2;
   }
   return 1;
+}
+
+// Both word alignments of the flat walk: 2 when either disagrees with the
+// walk, else the dword-aligned (LDS reader) verdict.
+template <int NS>
+static int flat_vs_walk_t(Handle *h, HostReader &rd, uint32_t i) {
+  HostBlockReader rb{rd};
+  const int a = flat_vs_walk_r<NS>(h, rd, i), b = flat_vs_walk_r<NS>(h, rb, i);
+  return a == 2 || b == 2 ? 2 : a;
 }
 
 extern "C" {

@@ -280,7 +280,7 @@ def test_deep_lines_walked_with_a_deep_stack():
 
 
 def _flat_line(rng):
-    keys = ["ts", "host", "status", "bytes", "path", "a", "", "k" * 40, "é", "status"]
+    keys = ["ts", "host", "status", "bytes", "path", "a", "", "k" * 40, "é", "status", "abcdefghijklmnopq"]
     parts = []
     for _ in range(rng.randrange(0, 9)):
         k = rng.choice(keys)
@@ -316,7 +316,11 @@ def test_flat_fast_path_never_disagrees_with_the_walk():
     # common shape (so the kernels use it).
     rng = random.Random(99)
     qs = ["count", "select(.status == 500) | count", "{h: .host, s: .status, b: .bytes}", ".a.b", ".[0]",
-          "select(.path.x == 1) | .k", 'select(.host == "x") | .ts', "group_by(.status) | sum(.bytes)"]
+          "select(.path.x == 1) | .k", 'select(.host == "x") | .ts', "group_by(.status) | sum(.bytes)",
+          # keys compared as words (empty, UTF-8, several lengths at once) and
+          # keys that are not (longer than 16 bytes, holding a quote)
+          '{a: .ts, b: .status, c: ."\u00e9", d: .""}', ".abcdefghijklmnopq", '."a\\"b"',
+          "{s: .status, l: .abcdefghijklmnopq}"]
     evs = [hostsim.Evaluator(q) for q in qs]
     accepted = 0
     for it in range(12000):

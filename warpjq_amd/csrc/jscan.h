@@ -1422,7 +1422,8 @@ WJQ_HD int eval_cond(R &rd, const PP &P, const SlotV (&sl)[NS]) {
 //     those closing quotes must be exactly the bytes before the colons.
 // Any byte outside that shape (whitespace, escapes, signs, fractions,
 // literals, nesting, control bytes, a line running past the first
-// (FLAT_WORDS - 1) x 64 + 32 bytes from its 16-byte block) makes it return false and the caller walks the line byte by byte,
+// (FLAT_WORDS - 1) x 64 + 32 bytes from its first byte rounded down to the
+// reader's flat_align) makes it return false and the caller walks the line byte by byte,
 // which decides everything; when it returns true the line is valid and the
 // captures of the fused paths are those walk() would take. Each lane runs it
 // on its own line, so there is no carry between lanes.
@@ -1522,6 +1523,21 @@ WJQ_HD uint32_t flat_next(const uint64_t (&m)[FLAT_WORDS], uint32_t from) {
   return r;
 }
 
+// Readers may declare `flat_align` (4 or 16) and `blk(pos)`, 16 bytes at
+// any multiple of it; the flat walk then starts its words at the line's
+// first byte rounded down to it, so fewer lines reach a fifth word. Others
+// are read in aligned 16-byte blocks.
+template <class R, class = void>
+struct flat_align : std::integral_constant<uint32_t, 16u> {};
+template <class R>
+struct flat_align<R, std::void_t<decltype(R::flat_align)>> : std::integral_constant<uint32_t, R::flat_align> {};
+
+template <class R>
+WJQ_HD Blk16 flat_blk(R &rd, uint32_t pos) {
+  if constexpr (flat_align<R>::value == 16u) return rd.block(pos >> 4);
+  else return rd.blk(pos);
+}
+
 template <class R, class = void>
 struct has_block_reads : std::false_type {};
 template <class R>
@@ -1549,7 +1565,7 @@ WJQ_HD void flat_word(R &rd, uint32_t B, uint32_t o0, uint32_t w, FlatCarry &F, 
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     if (j < NB) {
-      const Blk16 v = rd.block((B >> 4) + 4u * w + (uint32_t)j);
+      const Blk16 v = flat_blk(rd, B + 64u * w + 16u * (uint32_t)j);
       c[4 * j] = fclass4(v.x);
       c[4 * j + 1] = fclass4(v.y);
       c[4 * j + 2] = fclass4(v.z);
@@ -1586,8 +1602,8 @@ WJQ_HD void flat_word(R &rd, uint32_t B, uint32_t o0, uint32_t w, FlatCarry &F, 
   F.n_other += (uint32_t)__builtin_popcountll(oth);
   const uint64_t PQc = (Qc << 1) | F.pQc, PCM = (cm << 1) | F.pCM, PCL = (cl << 1) | F.pCL;
   const uint64_t PDG = (dg << 1) | F.pDG, POP = (OP << 1) | F.pOP;
-  const uint64_t KS = Qo & (PCM | POP), VS = Qo & PCL;
-  uint64_t bad = Qo & ~(KS | VS);
+  const uint64_t KS = Qo & (PCM | POP);
+  uint64_t bad = 0;
   // each key string's run, carried to its closing quote
   const uint64_t t = S + KS;
   const uint32_t c1 = t < S ? 1u : 0u;
@@ -1601,11 +1617,7 @@ WJQ_HD void flat_word(R &rd, uint32_t B, uint32_t o0, uint32_t w, FlatCarry &F, 
   bad |= (PCM & in) & ~Qo;
   bad |= (POP & in) & ~(Qo | oth);
   bad |= (PDG & in) & ~(dg | cm | oth);
-  bad |= cm & ~(PQc | PDG);
-  bad |= oth & ~(PQc | PDG | POP);
-  const uint64_t RS = dg & ~PDG;
-  bad |= RS & ~PCL;
-  const uint64_t ZRS = zr & RS;
+  const uint64_t ZRS = zr & ~PDG;  // zeros that start a digit run
   bad |= (((ZRS << 1) | F.pZR) & dg);  // a leading zero followed by a digit
   bad |= (PKE & in) ^ cl;              // colons are exactly the bytes after key ends
   F.bad |= bad;
@@ -1699,7 +1711,7 @@ WJQ_HD uint32_t flat_key_eq(R &rd, const PP &P, uint32_t cand, uint32_t x, uint3
 // written; walk the line.
 template <int NS, class R, class PP>
 WJQ_HD bool flat_walk(R &rd, uint32_t i, const PP &P, SlotV (&sl)[NS], uint32_t &te, uint32_t &end) {
-  const uint32_t B = i & ~15u;
+  const uint32_t B = i & ~(flat_align<R>::value - 1u);  // words start here
   const uint32_t o0 = i - B;
   uint64_t KSw[FLAT_WORDS], EDw[FLAT_WORDS], KEw[FLAT_WORDS];
   FlatCarry F;

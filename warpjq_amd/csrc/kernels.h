@@ -253,6 +253,13 @@ struct LdsReader {
     return *(const LDS_AS uint32_t *)(b + ((k << 2) + delta));
   }
   __device__ __forceinline__ uint32_t byte(uint32_t i) const { return b[i + delta]; }
+  // 16 bytes at any multiple of 4 (LDS reads need dword alignment only)
+  static constexpr uint32_t flat_align = 4u;
+  __device__ __forceinline__ Blk16 blk(uint32_t pos) const {
+    typedef unsigned int v4u_a4 __attribute__((ext_vector_type(4), aligned(4)));
+    v4u_a4 x = *(const LDS_AS v4u_a4 *)(b + (pos + delta));
+    return Blk16{x.x, x.y, x.z, x.w};
+  }
 };
 
 // Reads one line through a window of the wave's LDS arena that the whole
