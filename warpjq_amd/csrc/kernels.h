@@ -450,16 +450,21 @@ struct DevSink {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Exclusive prefix sum over the wave (every lane active), with DPP row
+// shifts and row broadcasts: no LDS round trips. Lanes a DPP step leaves out
+// (row/bank mask, or a source before the row) add 0.
 __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t &total) {
-  uint32_t lane = lane_id();
-  uint32_t x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    uint32_t y = __shfl_up(x, o, 64);
-    if (lane >= (uint32_t)o) x += y;
-  }
-  total = __shfl(x, 63, 64);
-  return x - v;
+  const int x = (int)v;
+  int t = x;
+  t += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true);  // row_shr:1
+  t += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true);  // row_shr:2
+  t += __builtin_amdgcn_update_dpp(0, x, 0x113, 0xF, 0xF, true);  // row_shr:3
+  t += __builtin_amdgcn_update_dpp(0, t, 0x114, 0xF, 0xE, true);  // row_shr:4, lanes 4-15 of a row
+  t += __builtin_amdgcn_update_dpp(0, t, 0x118, 0xF, 0xC, true);  // row_shr:8, lanes 8-15 of a row
+  t += __builtin_amdgcn_update_dpp(0, t, 0x142, 0xA, 0xF, true);  // row_bcast:15 into rows 1 and 3
+  t += __builtin_amdgcn_update_dpp(0, t, 0x143, 0xC, 0xF, true);  // row_bcast:31 into rows 2 and 3
+  total = (uint32_t)__builtin_amdgcn_readlane(t, 63);
+  return (uint32_t)t - v;
 }
 
 template <class T>
@@ -973,7 +978,8 @@ struct Claim {
 // the first pass). Lane l scans the contiguous slice [t0 + l*S, t0 + (l+1)*S)
 // of the staged claim (S = claim/64, 16..128 bytes), so one wave prefix sum
 // per pass orders every start. Advances c.resume / c.rank; *queued gets the
-// number queued and *last the highest start seen. Wave-uniform.
+// number queued and *last this lane's highest start (the caller reduces it
+// over the wave when it needs the claim's). Called wave-uniformly.
 __device__ __forceinline__ uint32_t enqueue_lines(const KArgs &a, const LDS_AS uint8_t *ar, LQueue &Q, uint32_t qtail,
                                                   uint32_t room, Claim &c, uint32_t *queued, uint32_t *last) {
   const uint32_t lane = lane_id();
@@ -1036,12 +1042,10 @@ __device__ __forceinline__ uint32_t enqueue_lines(const KArgs &a, const LDS_AS u
     }
     r++;
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) hi = max(hi, (uint32_t)__shfl_xor(hi, o, 64));
-  *last = hi;
+  *last = hi;  // this lane's; the caller reduces it over the wave when it needs it
   if (tot > room) {
     const uint64_t own = __ballot(next != c.end);
-    next = __shfl(next, (int)__builtin_ctzll(own), 64);
+    next = (uint32_t)__builtin_amdgcn_readlane((int)next, (int)__builtin_ctzll(own));
     c.resume = next;
     c.rank += room;
     *queued = room;
@@ -1117,7 +1121,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       const uint32_t hi = (uint32_t)(((uint64_t)(region + 1) * a.ntiles) / NREG);
       uint32_t tk = 0;
       if (lo < hi && lane == 0) tk = atomicAdd(&a.ctr->rticket[region * RSTRIDE], (unsigned)GRAB);
-      tk = __shfl(tk, 0, 64);
+      tk = (uint32_t)__builtin_amdgcn_readfirstlane((int)tk);
       if (lo + tk < hi) {
         if (lane == 0) {
           grab[0] = lo + tk + 1;
@@ -1225,6 +1229,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     const uint32_t total = enqueue_lines(a, ar, Q, qhead + qlen, QCAP - qlen, c, &got, &lst);
     if (!found && total) {
       // the claim's last line does not end inside the staged bytes
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) lst = max(lst, (uint32_t)__shfl_xor(lst, o, 64));
       c.last_unstaged = lst;
       if (got == total) {
         __syncthreads();
