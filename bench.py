@@ -244,7 +244,7 @@ def bench_served(args, w, np):
         if b > a:
             bounds.append((a, b))
         a = b
-    srv = w.Server(hbm_budget=200 << 30, workers=args.workers)
+    srv = w.Server(hbm_budget=200 << 30, workers=args.workers, chunk_bytes=args.served_chunk_mib << 20)
     ids = [srv.add_dataset(host[a:b]) for a, b in bounds]
     t0 = time.time()
     for h in [srv.submit("count", [i]) for i in ids]:  # load every dataset once
@@ -410,6 +410,7 @@ def main():
     ap.add_argument("--requests", type=int, default=600)
     ap.add_argument("--rates", type=float, nargs="+", default=[100.0, 500.0, 2000.0, 5000.0])
     ap.add_argument("--plain-requests", type=int, default=60, help="served: requests run one at a time, unserved")
+    ap.add_argument("--served-chunk-mib", type=int, default=64, help="served: device chunk of the server's engines")
     ap.add_argument("--files", type=int, default=4096)
     ap.add_argument("--long-frac", type=float, default=0.01, help="files mix: share of files with a long line")
     ap.add_argument("--graphs", action="store_true", help="files mix: also the per-file form with hipGraph replay")

@@ -220,6 +220,11 @@ typedef struct {
   uint64_t engines_opened, engines_reprogrammed; /* contexts created / reused for another query */
 } warpjq_amd_server_stats;
 
+/* chunk_bytes: the device chunk of the server's contexts (0: 64 MiB, at most
+ * 1 GiB); datasets are resident, so chunks past the reference's 64 MiB only
+ * cut launches per request. Each dataset keeps a newline index beside it in
+ * device memory (one byte per 64-byte block, counted in the budget) from
+ * which row requests count their lines. */
 warpjq_amd_server *warpjq_amd_server_create(uint64_t hbm_budget, uint32_t workers, uint64_t chunk_bytes, char *err,
                                             size_t err_len);
 void warpjq_amd_server_destroy(warpjq_amd_server *s);

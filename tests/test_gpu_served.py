@@ -10,7 +10,7 @@ import pytest
 import corpus
 import oracle
 import warpjq_amd as w
-from test_gpu_batched import chained
+from test_gpu_batched import BOM, chained
 
 pytestmark = pytest.mark.gpu
 
@@ -149,3 +149,35 @@ def test_reprogram_matches_fresh_context():
         ctx.close()
     finally:
         buf.free()
+
+
+def test_served_rows_count_lines_from_the_resident_index():
+    # A small chunk cuts datasets into pieces that start mid-dataset at any
+    # byte alignment, so row requests count each claim's lines from the
+    # datasets' resident newline indices plus the partial 64-byte blocks at
+    # the claim ends. Blank lines, CRLF, a BOM and a missing final newline:
+    # rows, line counts and strict-mode line numbers equal the oracle's.
+    data = [corpus.nginx(41, 3000), BOM + corpus.hostile(42, 600, 0.02),
+            corpus.hostile(43, 300, 0.0).rstrip(b"\n"), b"\n\n" + corpus.nginx(44, 500).replace(b"\n", b"\r\n"),
+            b"\n" * 5000 + corpus.nginx(45, 50)]
+    srv = w.Server(hbm_budget=1 << 30, workers=3, chunk_bytes=1 << 16)
+    try:
+        for d in data:
+            srv.add_dataset(d)
+        qs = ["select(.status >= 500) | {h: .host, s: .status}", "select(.status == 404)", ".msg"]
+        sets = [[0], [1, 2], [3, 0, 1], [4, 2], [2, 4, 3, 1, 0]]
+        handles = [(q, ds, srv.submit(q, ds)) for q in qs for ds in sets]
+        for q, ds, h in handles:
+            got, st, _ = srv.wait(h)
+            cpu, cst = oracle.run(q, chained([data[i] for i in ds]), chunk_bytes=1 << 16, threads=4)
+            assert got == cpu, (q, ds)
+            assert st["lines_in"] == cst["lines_in"], (q, ds)
+        # the first invalid line's number, past several pieces
+        bad = corpus.nginx(46, 2000)
+        cut = [i for i, c in enumerate(bad) if c == 10][1500] + 1
+        k = srv.add_dataset(bad[:cut] + b'{"status":}\n' + bad[cut:])
+        with pytest.raises(w.RunError) as e:
+            srv.wait(srv.submit("select(.status == 200)", [0, k], on_invalid=w.ABORT))
+        assert e.value.line == data[0].count(b"\n") + 1501 + 1
+    finally:
+        srv.close()

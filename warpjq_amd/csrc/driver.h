@@ -92,6 +92,10 @@ struct Engine {
   warpjq_ctx *ctx = nullptr;
   uint64_t cap = 0, max_lines = 0;
   bool csv = false;
+  // largest chunk engine_open accepts: the reference's 64 MiB device chunk
+  // (gpu/mod.rs:57); the served form, whose inputs are already resident,
+  // may raise it to cut launches per request
+  uint64_t cap_limit = 64ull << 20;
 };
 
 // 0 ok, 1 query error, 3 other error (message in err).
@@ -104,5 +108,19 @@ void engine_close(Engine &g);
 // Runs every batch `src` yields; the return codes of warpjq_amd_run_buffer.
 int engine_run(Engine &g, Source &src, int format, int on_invalid, uint8_t **out, uint64_t *out_len,
                warpjq_amd_stats *stats, char *err, size_t err_len);
+
+// Newline indices of resident datasets (warpjq_device.hip): newlines per
+// 64-byte block of the dataset, one byte each, kept beside it in device
+// memory. Row outputs over a span inside a registered dataset count each
+// claim's lines from the index instead of reading the whole claim.
+uint64_t nlindex_bytes(uint64_t n);
+// Builds the index of d[0..n) into idx (device memory, nlindex_bytes(n)
+// bytes); waits for it.
+bool nlindex_build(const uint8_t *d, uint64_t n, uint8_t *idx, std::string &err);
+void nlindex_register(const uint8_t *d, uint64_t n, const uint8_t *idx);
+void nlindex_unregister(const uint8_t *d);
+// The index of the registered dataset holding d[0..n), and d's offset in it
+// (null when none does).
+void nlindex_find(const uint8_t *d, uint64_t n, const uint8_t **idx, uint64_t *off);
 
 }  // namespace wjqd

@@ -864,7 +864,7 @@ int engine_open(Engine &g, const char *query, int format, uint64_t chunk_bytes, 
   g.csv = format == F_CSV;
   wjq::lower(g.prog, g.csv, g.low);
   g.view = g.low.view();
-  g.cap = std::max<uint64_t>(1, std::min<uint64_t>(chunk_bytes, 64ull << 20));
+  g.cap = std::max<uint64_t>(1, std::min<uint64_t>(chunk_bytes, g.cap_limit));
   char e[512] = {0};
   if (warpjq_ctx_create(&g.view, g.cap, 2, &g.ctx, e, sizeof e) != WARPJQ_OK) {
     snprintf(err, err_len, "creating the device context: %s", e);

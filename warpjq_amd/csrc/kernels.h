@@ -143,6 +143,11 @@ struct Span {
   uint32_t start;       // q position of its first byte
   uint32_t vend;        // q position one past its last byte
   uint32_t _pad;
+  // a resident dataset's newline index (newlines per 64-byte block of the
+  // dataset) when the span lies in one, else null; ioff is the dataset
+  // offset of the span's first byte
+  const uint8_t *nlidx;
+  uint64_t ioff;
 };
 
 struct KArgs {

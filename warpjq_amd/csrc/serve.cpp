@@ -48,7 +48,10 @@ struct Dataset {
   bool loading = false;
   uint32_t pins = 0;
   uint64_t last_use = 0;
-  uint64_t charge() const { return n + DEV_PAD; }
+  // the bytes, the kernels' pad, and the newline index (one byte per 64-byte
+  // block) that row requests count each claim's lines from
+  uint64_t idx_at() const { return (n + DEV_PAD + 255) & ~255ull; }
+  uint64_t charge() const { return idx_at() + wjqd::nlindex_bytes(n); }
 };
 
 struct Request {
@@ -108,6 +111,7 @@ bool make_room(warpjq_amd_server &s, uint64_t need, const Request &r) {
     }
     if (!victim) return false;
     // freed after the lock is released: hipFree waits for the device
+    wjqd::nlindex_unregister(victim->dev);
     s.to_free.push_back(victim->dev);
     victim->dev = nullptr;
     s.reserved -= victim->charge();
@@ -199,12 +203,15 @@ void worker(warpjq_amd_server *s) {
       uint8_t *p = nullptr;
       hipError_t e = hipMalloc((void **)&p, d.charge());
       if (e == hipSuccess && d.n) e = hipMemcpy(p, d.host, d.n, hipMemcpyHostToDevice);
+      std::string ierr;
+      const bool indexed = e == hipSuccess && wjqd::nlindex_build(p, d.n, p + d.idx_at(), ierr);
+      if (indexed) wjqd::nlindex_register(p, d.n, p + d.idx_at());
       std::lock_guard<std::mutex> g(s->mu);
       d.loading = false;
-      if (e != hipSuccess) {
+      if (!indexed) {
         if (p) (void)hipFree(p);
         s->reserved -= d.charge();
-        lerr = std::string("loading a dataset: ") + hipGetErrorString(e);
+        lerr = e != hipSuccess ? std::string("loading a dataset: ") + hipGetErrorString(e) : ierr;
       } else {
         d.dev = p;
         s->loads++;
@@ -255,6 +262,7 @@ void worker(warpjq_amd_server *s) {
       }
       engines.emplace_back();
       engines.back().key = key;
+      engines.back().eng.cap_limit = s->chunk_bytes;
       int rc = wjqd::engine_open(engines.back().eng, r->query.c_str(), r->fmt, s->chunk_bytes, err, sizeof err);
       if (rc) {
         engines.pop_back();
@@ -302,8 +310,8 @@ extern "C" {
 
 warpjq_amd_server *warpjq_amd_server_create(uint64_t hbm_budget, uint32_t workers, uint64_t chunk_bytes, char *err,
                                             size_t err_len) {
-  if (workers == 0 || workers > 16 || hbm_budget == 0) {
-    if (err && err_len) snprintf(err, err_len, "server_create: 1-16 workers and a non-zero budget");
+  if (workers == 0 || workers > 16 || hbm_budget == 0 || chunk_bytes > (1ull << 30)) {
+    if (err && err_len) snprintf(err, err_len, "server_create: 1-16 workers, a non-zero budget, chunks up to 1 GiB");
     return nullptr;
   }
   int n = 0;
@@ -329,7 +337,10 @@ void warpjq_amd_server_destroy(warpjq_amd_server *s) {
   for (uint8_t *p : s->to_free) (void)hipFree(p);
   for (auto &kv : s->reqs) free(kv.second->out);
   for (auto &d : s->ds)
-    if (d.dev) (void)hipFree(d.dev);
+    if (d.dev) {
+      wjqd::nlindex_unregister(d.dev);
+      (void)hipFree(d.dev);
+    }
   delete s;
 }
 

@@ -2,8 +2,10 @@
 // (gfx950): the non-template kernels, and the host side of the ABI. The
 // per-line kernels live in kernels.h and are instantiated per slot-table
 // width in kern_<N>.hip.
+#include <map>
 #include <mutex>
 
+#include "driver.h"
 #include "kernels.h"
 
 // ---------------------------------------------------------------------------
@@ -350,9 +352,25 @@ __global__ __launch_bounds__(1024) void k_res_prefix(unsigned long long *cnt, ui
   }
 }
 
+// This lane's share of the newlines at q positions [a, b) (d[q] is the byte
+// at q; 16-byte blocks dealt to the wave's lanes).
+__device__ __forceinline__ uint32_t nl_between(const uint8_t *d, uint32_t a, uint32_t b, uint32_t lane) {
+  uint32_t m32 = 0;
+  for (uint32_t k = (a >> 4) + lane; (k << 4) < b; k += 64) {
+    const uint32_t p = k << 4;
+    uint32_t m = nl_mask16(*(const uint4 *)(d + p));
+    if (p < a) m &= 0xFFFFu << (a - p);
+    if (p + 16u > b) m &= (1u << (b - p)) - 1u;
+    m32 += __popc(m);
+  }
+  return m32;
+}
+
 // Line starts per claim for row outputs, by the rule k_main's enqueue
 // applies: q in [max(t0, start), min(t0 + claim, vend)) with q == start or
-// byte q - 1 == '\n'. One wave per claim, 16 bytes per lane per step.
+// byte q - 1 == '\n'. One wave per claim, 16 bytes per lane per step. A span
+// inside a resident dataset with a newline index takes whole 64-byte blocks
+// from the index and reads only the partial blocks at the claim's two ends.
 __global__ __launch_bounds__(256) void k_line_count(const Span *spans, uint32_t n_spans, uint32_t ntiles, uint32_t claim,
                                                      unsigned long long *cnt) {
   const uint32_t lane = threadIdx.x & 63u;
@@ -367,16 +385,41 @@ __global__ __launch_bounds__(256) void k_line_count(const Span *spans, uint32_t 
       const uint32_t a0 = lo > sp.start ? lo - 1 : sp.start;  // newlines in [a0, a1) start a line
       const uint32_t a1 = hi - 1;
       uint32_t m32 = 0;
-      for (uint32_t b = (a0 >> 4) + lane; (b << 4) < a1; b += 64) {
-        const uint32_t p = b << 4;
-        uint32_t m = nl_mask16(*(const uint4 *)(d + p));
-        if (p < a0) m &= 0xFFFFu << (a0 - p);
-        if (p + 16u > a1) m &= (1u << (a1 - p)) - 1u;
-        m32 += __popc(m);
+      if (sp.nlidx) {
+        // dataset offsets of a0 and a1; whole index blocks [kA, kB) between
+        const uint64_t A = (uint64_t)(a0 - sp.start) + sp.ioff, B = (uint64_t)(a1 - sp.start) + sp.ioff;
+        const uint64_t kA = (A + 63u) >> 6, kB = B >> 6;
+        if (kA < kB) {
+          for (uint64_t k = kA + lane; k < kB; k += 64) m32 += sp.nlidx[k];
+          const uint32_t qA = (uint32_t)((kA << 6) - sp.ioff) + sp.start;
+          const uint32_t qB = (uint32_t)((kB << 6) - sp.ioff) + sp.start;
+          m32 += nl_between(d, a0, qA, lane) + nl_between(d, qB, a1, lane);
+        } else {
+          m32 = nl_between(d, a0, a1, lane);
+        }
+      } else {
+        m32 = nl_between(d, a0, a1, lane);
       }
       n = wave_sum((unsigned long long)m32) + (lo == sp.start ? 1u : 0u);
     }
     if (lane == 0) cnt[t] = n;
+  }
+}
+
+// Newlines per 64-byte block of d[0..n) (the last block partial).
+__global__ __launch_bounds__(256) void k_nl_index(const uint8_t *d, uint64_t n, uint8_t *idx) {
+  const uint64_t nb = (n + 63) >> 6;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nb; k += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) {
+      const uint64_t p = (k << 6) + 16u * j;
+      if (p >= n) break;
+      uint32_t m = nl_mask16(*(const uint4 *)(d + p));
+      if (p + 16u > n) m &= (1u << (uint32_t)(n - p)) - 1u;
+      c += __popc(m);
+    }
+    idx[k] = (uint8_t)c;
   }
 }
 
@@ -1084,12 +1127,15 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const warpjq_amd_spa
     sp.start = (uint32_t)(v + mis);
     sp.vend = (uint32_t)(v + mis + in[i].n_bytes);
     sp._pad = 0;
+    sp.nlidx = nullptr;
+    sp.ioff = 0;
+    if (ctx->rows) wjqd::nlindex_find(in[i].d_bytes, in[i].n_bytes, &sp.nlidx, &sp.ioff);
     s.h_spans[ns++] = sp;
     total += in[i].n_bytes;
     v = ((v + mis + in[i].n_bytes + claim - 1) / claim) * claim;
   }
   if (ns == 0) {  // keep one empty span so results can be read uniformly
-    s.h_spans[0] = Span{in && n_in ? in[0].d_bytes : nullptr, 0, 0, 0, 0};
+    s.h_spans[0] = Span{in && n_in ? in[0].d_bytes : nullptr, 0, 0, 0, 0, nullptr, 0};
   }
   s.n_spans = ns ? ns : 1;
   const uint32_t ntiles = (uint32_t)(v / claim);
@@ -1686,7 +1732,7 @@ warpjq_status warpjq_amd_aggregate_device_noalloc(const warpjq_program *prog, co
     return WARPJQ_ERR_INVALID_ARG;
   }
   hipStream_t st = 0;
-  Span sp{d_bytes ? d_bytes - mis : nullptr, 0, mis, (uint32_t)vend, 0};
+  Span sp{d_bytes ? d_bytes - mis : nullptr, 0, mis, (uint32_t)vend, 0, nullptr, 0};
   HIP_TRY(hipMemcpyAsync(S + L.tab, buf.data(), buf.size(), hipMemcpyHostToDevice, st), "H2D(program)");
   HIP_TRY(hipMemcpyAsync(S + L.span, &sp, sizeof sp, hipMemcpyHostToDevice, st), "H2D(span)");
   DevCtr *ctr = (DevCtr *)(S + L.ctr);
@@ -1776,3 +1822,66 @@ warpjq_status warpjq_amd_aggregate_device_noalloc(const warpjq_program *prog, co
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// resident newline indices (the served form's datasets)
+// ---------------------------------------------------------------------------
+
+namespace wjqd {
+
+namespace {
+struct NlIndex {
+  uint64_t n;
+  const uint8_t *idx;
+};
+std::mutex g_nl_mu;
+std::map<uintptr_t, NlIndex> g_nl;  // by dataset base address
+}  // namespace
+
+uint64_t nlindex_bytes(uint64_t n) { return (n + 63) / 64; }
+
+bool nlindex_build(const uint8_t *d, uint64_t n, uint8_t *idx, std::string &err) {
+  if (!n) return true;
+  const uint64_t nb = nlindex_bytes(n);
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(4096, (nb + 255) / 256);
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+    err = "hipStreamCreate (index)";
+    return false;
+  }
+  hipLaunchKernelGGL(k_nl_index, dim3(grid), dim3(256), 0, st, d, n, idx);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipStreamDestroy(st);
+  if (e != hipSuccess) {
+    err = std::string("building a newline index: ") + hipGetErrorString(e);
+    return false;
+  }
+  return true;
+}
+
+void nlindex_register(const uint8_t *d, uint64_t n, const uint8_t *idx) {
+  std::lock_guard<std::mutex> g(g_nl_mu);
+  g_nl[(uintptr_t)d] = NlIndex{n, idx};
+}
+
+void nlindex_unregister(const uint8_t *d) {
+  std::lock_guard<std::mutex> g(g_nl_mu);
+  g_nl.erase((uintptr_t)d);
+}
+
+void nlindex_find(const uint8_t *d, uint64_t n, const uint8_t **idx, uint64_t *off) {
+  *idx = nullptr;
+  *off = 0;
+  std::lock_guard<std::mutex> g(g_nl_mu);
+  auto it = g_nl.upper_bound((uintptr_t)d);
+  if (it == g_nl.begin()) return;
+  --it;
+  const uintptr_t base = it->first;
+  if ((uintptr_t)d + n <= base + it->second.n) {
+    *idx = it->second.idx;
+    *off = (uint64_t)((uintptr_t)d - base);
+  }
+}
+
+}  // namespace wjqd
