@@ -13,7 +13,9 @@ a bounded sample, host threads).
 lognormal sizes clamped to [4 KB, 256 MB], 16 GiB in all, 1% of them holding
 a line of up to 1 MB, resident in HBM; the filter-count over all of them
 once per file (one launch each, two slots in flight) and batched (spans of
-many files per launch, warpjq_amd_submit_spans).
+many files per launch, warpjq_amd_submit_spans); then the mix's CSV slice:
+`--csv` output of a projection over its first 512 MB of files, resident,
+compared byte for byte with the oracle and timed.
 
 `--workload served` measures the served form: 64 nginx datasets of
 log-uniform sizes in [1 MB, 2 GB] kept resident by a warpjq_amd server
@@ -62,7 +64,7 @@ def file_sizes(rng, n_files, total, lo=4000, hi=256_000_000):
     return s.astype(np.int64)
 
 
-def bench_files(args, w, torch, np):
+def bench_files(args, w, torch, np, oracle):
     n_files = args.files
     total = int(args.size_gib * (1 << 30)) if args.size_gib != 8.0 else 16 << 30
     rng = np.random.default_rng(args.seed)
@@ -178,6 +180,9 @@ def bench_files(args, w, torch, np):
         res[name] = {"ms": round(best * 1e3, 2), "gbps": round(n / best / 1e9, 1)}
         log(name, res[name])
     speedup = res["per_file"]["ms"] / res["batched"]["ms"]
+    del d
+    torch.cuda.empty_cache()
+    csv = csv_slice(args, w, oracle, host, bounds)
     line = {
         "metric": "input GB/s of select(.status == 500) | count over a 4,096-file NDJSON mix resident in HBM "
                   "(batched form)",
@@ -202,8 +207,46 @@ def bench_files(args, w, torch, np):
         "speedup_batched_vs_per_file": round(speedup, 2),
         "batched_main_ms_per_launch": launch_ms,
         "per_file_graphs": res.get("per_file_graphs"),
+        "csv_slice": csv,
     }
     print(json.dumps(line), flush=True)
+
+
+CSV_QUERY = "select(.status >= 500) | {host: .host, status: .status, path: .path, bytes: .bytes}"
+
+
+def csv_slice(args, w, oracle, host, bounds):
+    """The mix's CSV slice: --csv output of a projection over its first
+    512 MB of files, resident (the served form, one request over all of them
+    read as one stream), checked byte for byte against the oracle before the
+    timed runs."""
+    files, size = [], 0
+    for a, b in bounds:
+        if size >= 512_000_000:
+            break
+        files.append(host[a:b].tobytes())
+        size += b - a
+    srv = w.Server(hbm_budget=4 << 30, workers=1)
+    try:
+        ids = [srv.add_dataset(f) for f in files]
+        out, st, _ = srv.wait(srv.submit(CSV_QUERY, ids, fmt=w.FMT_CSV))
+        cpu, _ = oracle.run(CSV_QUERY, b"".join(files), fmt=oracle.FMT_CSV, threads=min(16, os.cpu_count() or 1),
+                            chunk_bytes=256 << 20)
+        if out != cpu:
+            raise SystemExit("parity: the CSV slice differs from the oracle")
+        best = None
+        for _ in range(max(1, args.steps)):
+            t = time.perf_counter()
+            out, st, _ = srv.wait(srv.submit(CSV_QUERY, ids, fmt=w.FMT_CSV))
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        r = {"query": CSV_QUERY, "files": len(files), "bytes": size, "rows": out.count(b"\n") - 1,
+             "out_bytes": len(out), "ms": round(best * 1e3, 2), "gbps": round(size / best / 1e9, 1),
+             "identical_to_oracle": True}
+        log("csv slice", r)
+        return r
+    finally:
+        srv.close()
 
 
 SERVED_HOSTS = ["web-01.prod.us-east-1.example.com", "api-1.prod.eu-west-2.internal",
@@ -440,7 +483,7 @@ def main():
     w.probe()
     dev = w.device_name()
     if args.workload == "files":
-        return bench_files(args, w, torch, np)
+        return bench_files(args, w, torch, np, oracle)
     if args.workload == "served":
         return bench_served(args, w, np)
     if args.workload == "nested":

@@ -400,6 +400,21 @@ static int drain(Run &r, uint32_t slot, const Batch &b, std::vector<Batch> &redo
 // Newline-aligned chunks of at most `cap` bytes holding at most `max_lines`
 // lines (gpu/mod.rs:697-810). When the line at `pos` alone is longer than
 // `cap`, returns false with *too_long set.
+// Newlines in p[0..n), found with memchr (vectorised in the C library; the
+// per-byte loop it replaces ran at a few GB/s and, over a batch of many small
+// files, left the GPU idle between chunks).
+static uint64_t count_nl(const uint8_t *p, uint64_t n) {
+  uint64_t c = 0;
+  const uint8_t *e = p + n;
+  while (p < e) {
+    const uint8_t *q = (const uint8_t *)memchr(p, '\n', (size_t)(e - p));
+    if (!q) break;
+    c++;
+    p = q + 1;
+  }
+  return c;
+}
+
 static bool next_chunk(const uint8_t *d, uint64_t n, uint64_t &pos, uint64_t cap, uint64_t max_lines,
                        uint64_t max_line, Chunk &c, std::string &err, bool *too_long = nullptr) {
   if (pos >= n) return false;
@@ -417,28 +432,36 @@ static bool next_chunk(const uint8_t *d, uint64_t n, uint64_t &pos, uint64_t cap
     end = (uint64_t)(nl - d) + 1;
   }
   // line budget: count only when the head of the chunk is dense
-  uint64_t sample = std::min<uint64_t>(1 << 16, end - pos);
-  uint64_t seen = 0;
-  for (uint64_t i = 0; i < sample; i++) seen += d[pos + i] == '\n';
+  const uint64_t sample = std::min<uint64_t>(1 << 16, end - pos);
+  const uint64_t seen = count_nl(d + pos, sample);
   uint64_t projected = sample ? seen * (end - pos) / sample : 0;
   if (projected * 2 >= max_lines) {
     uint64_t lines = 0;
-    for (uint64_t i = pos; i < end; i++)
-      if (d[i] == '\n' && ++lines == max_lines) {
-        if (i + 1 < end) end = i + 1;
+    const uint8_t *q = d + pos, *e = d + end;
+    while (q < e) {
+      const uint8_t *nl = (const uint8_t *)memchr(q, '\n', (size_t)(e - q));
+      if (!nl) break;
+      if (++lines == max_lines) {
+        if (nl + 1 < e) end = (uint64_t)(nl - d) + 1;
         break;
       }
+      q = nl + 1;
+    }
   }
   if (max_line < cap) {
     uint64_t s = pos;
-    for (uint64_t i = pos; i < end; i++)
-      if (d[i] == '\n') {
-        if (i - s > max_line) {
-          err = "a line exceeds the " + std::to_string(max_line) + " byte limit";
-          return false;
-        }
-        s = i + 1;
+    const uint8_t *q = d + pos, *e = d + end;
+    while (q < e) {
+      const uint8_t *nl = (const uint8_t *)memchr(q, '\n', (size_t)(e - q));
+      if (!nl) break;
+      const uint64_t i = (uint64_t)(nl - d);
+      if (i - s > max_line) {
+        err = "a line exceeds the " + std::to_string(max_line) + " byte limit";
+        return false;
       }
+      s = i + 1;
+      q = nl + 1;
+    }
     if (end - s > max_line && (end == n || d[end - 1] != '\n')) {
       err = "a line exceeds the " + std::to_string(max_line) + " byte limit";
       return false;
@@ -605,8 +628,8 @@ bool Batcher::next(Batch &b, std::string &err) {
     }
     // line budget for the whole batch (the kernels report an overflow
     // otherwise, and the batch is split)
-    uint64_t sample = std::min<uint64_t>(1 << 16, c.len), seen = 0;
-    for (uint64_t i = 0; i < sample; i++) seen += d[c.off + i] == '\n';
+    const uint64_t sample = std::min<uint64_t>(1 << 16, c.len);
+    const uint64_t seen = count_nl(d + c.off, sample);
     const uint64_t est = sample ? seen * c.len / sample + 1 : 1;
     if (!b.empty() && (lines + est) * 2 >= max_lines) break;
     lines += est;
