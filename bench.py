@@ -591,12 +591,19 @@ def main():
             extras[q] = {"kernel_gbps": round(n / (chain / 1e3) / 1e9, 1), "rows": rows,
                          "alg_gbps": round((n + out_bytes) / (chain / 1e3) / 1e9, 1)}
             log(q, extras[q])
-        # end to end from a host buffer (pinned staging + H2D + kernels + D2H)
-        t = time.perf_counter()
-        out, st = w.run(QUERY, host, chunk_bytes=64 << 20, n=n)
-        e2e = time.perf_counter() - t
-        assert int(out) == hist[500]
-        extras["e2e"] = {"gbps": round(n / e2e / 1e9, 2), "h2d_ms": round(st["h2d_ms"], 1),
+        # end to end from a host buffer (pinned staging + H2D + kernels + D2H):
+        # the first call in the process also starts the staging threads and
+        # pins their buffers; the second is the steady state reported
+        e2e_runs = []
+        for _ in range(2):
+            t = time.perf_counter()
+            out, st = w.run(QUERY, host, chunk_bytes=64 << 20, n=n)
+            e2e_runs.append(time.perf_counter() - t)
+            if int(out) != hist[500]:
+                raise SystemExit("parity: the end-to-end count differs from the generator's")
+        e2e = e2e_runs[-1]
+        extras["e2e"] = {"gbps": round(n / e2e / 1e9, 2), "gbps_first_call": round(n / e2e_runs[0] / 1e9, 2),
+                         "h2d_ms": round(st["h2d_ms"], 1),
                          "kernel_ms": round(st["kernel_ms"], 1), "chunks": st["chunks"],
                          "h2d_gbps": round(n / (st["h2d_ms"] / 1e3) / 1e9, 1) if st["h2d_ms"] else None}
         log("e2e", extras["e2e"])
