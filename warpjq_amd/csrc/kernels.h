@@ -1141,8 +1141,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   };
   // Starts the direct-to-LDS copy of claim t (and the first 256 bytes past
   // it) to arena offset `at`; returns whether a line starts at its first
-  // byte (read through the scalar cache; waited for on use).
-  auto issue_fetch = [&](uint32_t t, uint32_t at) -> uint32_t {
+  // byte (read through the scalar cache; waited for on use). Row outputs
+  // also start loading the claim's first line index into `pre`, so its
+  // latency overlaps the copy's.
+  auto issue_fetch = [&](uint32_t t, uint32_t at, unsigned long long &pre) -> uint32_t {
     const uint32_t t0 = t * a.claim;
     const Span sp = a.spans[span_of(a.spans, a.n_spans, t0)];
     const uint8_t *src = sp.base + (t0 - sp.vbase);
@@ -1153,6 +1155,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     }
     uint32_t carry0 = 1;  // a span's first claim: its first line start is placed by c.start
     if (t0 > sp.vbase) carry0 = (*(const uint32_t *)(src - 4) >> 24) == '\n' ? 1u : 0u;
+    if (LB) pre = a.tile_state[t];
     return carry0;
   };
   // Keeps the bytes of queued lines at the front of the arena.
@@ -1181,7 +1184,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   };
   // Waits for claim t's bytes at `at`, stages its last line's end, and queues
   // its line starts.
-  auto process_claim = [&](uint32_t t, uint32_t at, uint32_t carry0) {
+  auto process_claim = [&](uint32_t t, uint32_t at, uint32_t carry0, unsigned long long pre) {
     const uint32_t t0 = t * a.claim;
     const Span sp = a.spans[span_of(a.spans, a.n_spans, t0)];
     const uint32_t vend = sp.vend;
@@ -1244,7 +1247,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       }
     }
     lines_total += total;
-    if (LB) c.base = (uint32_t)a.tile_state[t];  // the pre-pass's exclusive prefix
+    if (LB) c.base = (uint32_t)pre;  // the pre-pass's exclusive prefix
     else if (lane == 0) a.tile_state[t] = total;
     if (total) {
       if (lane == 0) {
@@ -1263,6 +1266,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 
   bool pf = false;  // a claim's copy was started before the last round
   uint32_t pf_t = 0, pf_at = 0, pf_carry0 = 0;
+  unsigned long long pf_pre = 0;
 
   for (;;) {
     // ---- fill the queue to one full round when possible ----
@@ -1279,7 +1283,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       }
       if (pf) {
         pf = false;
-        process_claim(pf_t, pf_at, pf_carry0);
+        process_claim(pf_t, pf_at, pf_carry0, pf_pre);
         continue;
       }
       if (!tiles_left) break;
@@ -1291,8 +1295,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
         tiles_left = false;
         break;
       }
-      const uint32_t carry0 = issue_fetch(t, used);
-      process_claim(t, used, carry0);
+      unsigned long long pre = 0;
+      const uint32_t carry0 = issue_fetch(t, used, pre);
+      process_claim(t, used, carry0, pre);
     }
     if (qlen == 0) {
       if (!pending && !tiles_left && !pf) break;
@@ -1309,7 +1314,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
           pf = true;
           pf_t = t;
           pf_at = used;
-          pf_carry0 = issue_fetch(t, used);
+          pf_carry0 = issue_fetch(t, used, pf_pre);
         }
       }
     }
@@ -1502,17 +1507,51 @@ __global__ __launch_bounds__(256) void k_emit(const Span *spans, uint32_t n_span
   uint64_t nsel = ctr->n_sel;
   if (P.output_kind == WARPJQ_OUT_PASSTHROUGH && !P.csv_mode) {
     // a whole-line row is the line's bytes up to a final '\r', then '\n':
-    // one wave copies a row, 64 contiguous bytes per step, no re-walk
+    // no re-walk. A wave takes 64 rows: each lane loads one row's start and
+    // output range (one round of dependent loads for all 64), then the wave
+    // copies the rows one after another, 64 contiguous bytes per lane step,
+    // a row's loads issued before its stores.
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    for (uint64_t r = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; r < nsel; r += nw) {
-      const uint32_t s = sel_start[sel_idx[r]];
-      const uint64_t o = row_off[r];
-      const uint32_t len = (uint32_t)(row_off[r + 1] - o) - 1u;
-      const Span sp = spans[span_of(spans, n_spans, s)];
-      const uint8_t *src = sp.base + (s - sp.vbase);
-      for (uint32_t k = lane; k < len; k += 64) out[o + k] = src[k];
-      if (lane == 0) out[o + len] = '\n';
+    uint32_t sv0 = 0, sv1 = 0;  // the span last used: q positions [sv0, sv1)
+    const uint8_t *sbase = nullptr;
+    for (uint64_t r0 = (((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 64u; r0 < nsel; r0 += nw * 64u) {
+      const uint64_t r = r0 + lane;
+      uint32_t s = 0, len = 0;
+      uint64_t o = 0;
+      if (r < nsel) {
+        s = sel_start[sel_idx[r]];
+        o = row_off[r];
+        len = (uint32_t)(row_off[r + 1] - o) - 1u;
+      }
+      const uint32_t m = (uint32_t)min<uint64_t>(64u, nsel - r0);
+      for (uint32_t k = 0; k < m; k++) {
+        const uint32_t sk = (uint32_t)__builtin_amdgcn_readlane((int)s, (int)k);
+        const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)len, (int)k);
+        const uint64_t ok = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(o >> 32), (int)k) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)o, (int)k);
+        if (sk < sv0 || sk >= sv1) {
+          const Span sp = spans[span_of(spans, n_spans, sk)];
+          sv0 = sp.vbase;
+          sv1 = sp.vend;
+          sbase = sp.base - sp.vbase;
+        }
+        const uint8_t *src = sbase + sk;
+        for (uint32_t j0 = 0; j0 < lk; j0 += 256u) {
+          uint8_t b[4];
+#pragma unroll
+          for (uint32_t u = 0; u < 4; u++) {
+            const uint32_t j = j0 + 64u * u + lane;
+            b[u] = j < lk ? src[j] : (uint8_t)0;
+          }
+#pragma unroll
+          for (uint32_t u = 0; u < 4; u++) {
+            const uint32_t j = j0 + 64u * u + lane;
+            if (j < lk) out[ok + j] = b[u];
+          }
+        }
+        if (lane == 0) out[ok + lk] = '\n';
+      }
     }
     return;
   }
