@@ -1718,7 +1718,7 @@ WJQ_HD bool flat_walk(R &rd, uint32_t i, const PP &P, SlotV (&sl)[NS], uint32_t 
 #pragma unroll
   for (int w = 0; w < FLAT_WORDS; w++) {
     KSw[w] = EDw[w] = KEw[w] = 0;
-    if (F.ended) continue;
+    if (F.ended || F.bad || F.bsl || F.n_other > 1) continue;  // decided: valid end, or walk it
     if (w + 1 < FLAT_WORDS) flat_word<4>(rd, B, o0, (uint32_t)w, F, KSw[w], KEw[w], EDw[w]);
     else flat_word<2>(rd, B, o0, (uint32_t)w, F, KSw[w], KEw[w], EDw[w]);
   }
