@@ -2,11 +2,18 @@
 // (gfx950): the non-template kernels, and the host side of the ABI. The
 // per-line kernels live in kernels.h and are instantiated per slot-table
 // width in kern_<N>.hip.
+#include <atomic>
 #include <map>
 #include <mutex>
 
 #include "driver.h"
 #include "kernels.h"
+
+namespace wjqd {
+// Sets each span's newline index and dataset offset when it lies inside a
+// registered resident dataset (defined with the registry below).
+void nlindex_attach(Span *sp, uint32_t n);
+}  // namespace wjqd
 
 // ---------------------------------------------------------------------------
 // k_reset
@@ -1129,7 +1136,6 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const warpjq_amd_spa
     sp._pad = 0;
     sp.nlidx = nullptr;
     sp.ioff = 0;
-    if (ctx->rows) wjqd::nlindex_find(in[i].d_bytes, in[i].n_bytes, &sp.nlidx, &sp.ioff);
     s.h_spans[ns++] = sp;
     total += in[i].n_bytes;
     v = ((v + mis + in[i].n_bytes + claim - 1) / claim) * claim;
@@ -1138,6 +1144,9 @@ static warpjq_status launch_chain(warpjq_ctx *ctx, Slot &s, const warpjq_amd_spa
     s.h_spans[0] = Span{in && n_in ? in[0].d_bytes : nullptr, 0, 0, 0, 0, nullptr, 0};
   }
   s.n_spans = ns ? ns : 1;
+  // row outputs over spans inside resident datasets count their lines from
+  // the datasets' newline indices (one registry lookup pass per submit)
+  if (ctx->rows && ns) wjqd::nlindex_attach(s.h_spans, ns);
   const uint32_t ntiles = (uint32_t)(v / claim);
   s.ntiles = ntiles;
   // a resident submit seen before on this slot: replay its captured chain
@@ -1836,6 +1845,7 @@ struct NlIndex {
 };
 std::mutex g_nl_mu;
 std::map<uintptr_t, NlIndex> g_nl;  // by dataset base address
+std::atomic<uint32_t> g_nl_n{0};    // entries in g_nl
 }  // namespace
 
 uint64_t nlindex_bytes(uint64_t n) { return (n + 63) / 64; }
@@ -1863,17 +1873,19 @@ bool nlindex_build(const uint8_t *d, uint64_t n, uint8_t *idx, std::string &err)
 void nlindex_register(const uint8_t *d, uint64_t n, const uint8_t *idx) {
   std::lock_guard<std::mutex> g(g_nl_mu);
   g_nl[(uintptr_t)d] = NlIndex{n, idx};
+  g_nl_n.store((uint32_t)g_nl.size());
 }
 
 void nlindex_unregister(const uint8_t *d) {
   std::lock_guard<std::mutex> g(g_nl_mu);
   g_nl.erase((uintptr_t)d);
+  g_nl_n.store((uint32_t)g_nl.size());
 }
 
-void nlindex_find(const uint8_t *d, uint64_t n, const uint8_t **idx, uint64_t *off) {
+// The registered dataset holding d[0..n) (caller holds g_nl_mu).
+static void nlindex_find_locked(const uint8_t *d, uint64_t n, const uint8_t **idx, uint64_t *off) {
   *idx = nullptr;
   *off = 0;
-  std::lock_guard<std::mutex> g(g_nl_mu);
   auto it = g_nl.upper_bound((uintptr_t)d);
   if (it == g_nl.begin()) return;
   --it;
@@ -1881,6 +1893,20 @@ void nlindex_find(const uint8_t *d, uint64_t n, const uint8_t **idx, uint64_t *o
   if ((uintptr_t)d + n <= base + it->second.n) {
     *idx = it->second.idx;
     *off = (uint64_t)((uintptr_t)d - base);
+  }
+}
+
+void nlindex_find(const uint8_t *d, uint64_t n, const uint8_t **idx, uint64_t *off) {
+  std::lock_guard<std::mutex> g(g_nl_mu);
+  nlindex_find_locked(d, n, idx, off);
+}
+
+void nlindex_attach(Span *sp, uint32_t n) {
+  if (g_nl_n.load() == 0) return;
+  std::lock_guard<std::mutex> g(g_nl_mu);
+  for (uint32_t i = 0; i < n; i++) {
+    const uint8_t *d = sp[i].base + (sp[i].start - sp[i].vbase);  // the span's first byte
+    nlindex_find_locked(d, sp[i].vend - sp[i].start, &sp[i].nlidx, &sp[i].ioff);
   }
 }
 
