@@ -1818,11 +1818,12 @@ struct LineResult {
   uint32_t status;
   uint32_t selected;  // passed the filter and every post path resolved
   uint32_t end;       // position of the terminator (valid lines)
+  uint32_t flat;      // the bit-parallel fast path decided the line
 };
 
 template <int NS, class R, class PP>
-WJQ_HD LineResult eval_line(R &rd, uint32_t s, const PP &P, SlotV (&sl)[NS]) {
-  LineResult r{ST_OK, 0, 0};
+WJQ_HD LineResult eval_line(R &rd, uint32_t s, const PP &P, SlotV (&sl)[NS], bool try_flat = true) {
+  LineResult r{ST_OK, 0, 0, 0};
   uint32_t i = s;
   bool ff = false;
   for (;;) {
@@ -1853,7 +1854,8 @@ WJQ_HD LineResult eval_line(R &rd, uint32_t s, const PP &P, SlotV (&sl)[NS]) {
   uint32_t te = 0;
   int st;
   if constexpr (has_block_reads<R>::value && !wide_reader<R>::value) {
-    st = (rd.byte(i) == '{' && flat_walk<NS>(rd, i, P, sl, te, r.end)) ? (int)ST_OK : walk<NS>(rd, i, P, sl, te, r.end);
+    r.flat = try_flat && rd.byte(i) == '{' && flat_walk<NS>(rd, i, P, sl, te, r.end);
+    st = r.flat ? (int)ST_OK : walk<NS>(rd, i, P, sl, te, r.end);
   } else {
     st = walk<NS>(rd, i, P, sl, te, r.end);
   }

@@ -84,6 +84,7 @@ using namespace wjqd;
 #define GROUP_ARENA_DEFAULT 14336u
 #define ARENA_MAX 32768u             // offsets fit 15 bits; bit 15 marks unstaged lines
 #define QCAP 128u                    // per-wave LDS queue of line starts
+#define FLAT_SKIP 16u                // rounds without the flat fast path after a round with no flat line
 #define QMASK (QCAP - 1u)
 #define LGROUP 32u                   // per-wave LDS group table entries
 #define GROUP_MAX_PROBE 128u
@@ -791,10 +792,12 @@ __device__ __forceinline__ void acc_merge(LaneAcc &L, const LaneAcc &d) {
 // wave-wide phase, whose reader decides it.
 template <int NS, bool GRP, bool LB, class R>
 __device__ __forceinline__ bool process_line(R &rd, const KArgs &a, const LProg &P, uint32_t s, uint32_t rec,
-                                             uint32_t tile, LaneAcc &L, LGroup *lgp, bool act = true) {
+                                             uint32_t tile, LaneAcc &L, LGroup *lgp, bool act = true,
+                                             bool try_flat = true, uint32_t *flat = nullptr) {
   constexpr bool WIDE = wide_reader<R>::value;
   SlotV sl[NS];
-  LineResult r = eval_line<NS>(rd, s, P, sl);
+  LineResult r = eval_line<NS>(rd, s, P, sl, try_flat);
+  if (flat) *flat = r.flat;
   if (r.status == ST_OK && r.selected && P.output_kind == WARPJQ_OUT_AGG) {
     bool has_num = false, is_int = false, undecided = false;
     double v = 0.0;
@@ -1264,6 +1267,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     __syncthreads();
   };
 
+  uint32_t flat_skip = 0;  // rounds left that skip the flat fast path (wave-uniform)
   bool pf = false;  // a claim's copy was started before the last round
   uint32_t pf_t = 0, pf_at = 0, pf_carry0 = 0;
   unsigned long long pf_pre = 0;
@@ -1321,6 +1325,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 
     // ---- one round: lane k evaluates queued line k ----
     const uint32_t take = qlen < 64 ? qlen : 64;
+    // after a full-ish round in which no line was a flat object, the next
+    // FLAT_SKIP rounds go straight to the byte walk (the same results: the
+    // walk decides every line the fast path does)
+    const bool try_flat = flat_skip == 0;
+    uint32_t flat_hit = 0;
     if (lane < take) {
       const uint32_t e = (qhead + lane) & QMASK;
       const uint32_t qv = Q.qe[e];
@@ -1333,7 +1342,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       bool wide = (qv & QE_UNSTAGED) != 0;  // too long to stage
       if (!wide) {
         LdsReader rd(ar, off - s);
-        wide = process_line<NS, GRP, LB>(rd, a, P, s, rec, tile, L, lgp);  // or declined per lane
+        wide = process_line<NS, GRP, LB>(rd, a, P, s, rec, tile, L, lgp, true, try_flat, &flat_hit);
       }
       if (wide) {
         // walked by the whole wave after the claims; a full queue makes the
@@ -1348,6 +1357,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
           L.ovf = 1;
         }
       }
+    }
+    if (try_flat) {
+      if (take >= 32 && __ballot(flat_hit != 0) == 0) flat_skip = FLAT_SKIP;
+    } else {
+      flat_skip--;
     }
     qhead = (qhead + take) & QMASK;
     qlen -= take;
