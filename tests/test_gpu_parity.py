@@ -89,6 +89,25 @@ def test_nginx_filter_count():
         assert gst["lines_in"] == cst["lines_in"]
 
 
+def test_flat_lines_after_runs_of_nested_ones():
+    # A wave that sees a round with no flat line walks the next rounds byte by
+    # byte; flat lines arriving then (and long runs of either kind, in
+    # several chunks) give the oracle's result whichever path takes them.
+    nested = corpus.hostile(12, 3000, 0.02).split(b"\n")[:-1]
+    flat = corpus.nginx(13, 9000).split(b"\n")[:-1]
+    lines = []
+    for k in range(6):
+        lines += nested[500 * k:500 * (k + 1)]
+        lines += flat[1500 * k:1500 * (k + 1)]
+    data = b"\n".join(lines) + b"\n"
+    for q in ("select(.status == 500) | count", "group_by(.host) | count", "select(.status >= 500) | {h: .host, s: .status}",
+              "sum(.bytes)", "count"):
+        for chunk in (1 << 16, 1 << 20):
+            cpu, cst, gpu, gst = both(q, data, chunk=chunk)
+            assert gpu == cpu, (q, chunk)
+            assert gst["lines_in"] == cst["lines_in"] and gst["malformed"] == cst["malformed"], (q, chunk)
+
+
 def long_lines_corpus(seed):
     """Short lines around lines of 9 KB - 1.2 MB: long strings, long arrays,
     deep objects, a long invalid line and a long line whose key is late."""
