@@ -192,6 +192,19 @@ int warpjq_amd_run_files(const char *query, const uint8_t *const *files, const u
                          int format, int on_invalid, uint64_t chunk_bytes, uint64_t max_line_bytes, uint8_t **out,
                          uint64_t *out_len, warpjq_amd_stats *stats, char *err, size_t err_len);
 
+/* The same over a CSV input, which the device transcodes into NDJSON
+ * records first: the header line names the columns and reads as a blank
+ * line; every later line is the object {"name":value,...} (an empty unquoted
+ * field null, an unquoted JSON number a number, anything else a string; ""
+ * inside quotes is a quote; a trailing CR dropped); a line with bad quoting
+ * or another field count than the header's is one malformed line. No
+ * counterpart in the reference, which reads NDJSON only. Same arguments and
+ * return codes as warpjq_amd_run_buffer; stats->kernel_ms includes the
+ * transcoding kernels. */
+int warpjq_amd_run_csv(const char *query, const uint8_t *data, uint64_t n, int format, int on_invalid,
+                       uint64_t chunk_bytes, uint64_t max_line_bytes, uint8_t **out, uint64_t *out_len,
+                       warpjq_amd_stats *stats, char *err, size_t err_len);
+
 /* The same over files named by path, as the reference's CLI opens its inputs
  * (Input::open / Input::stdin / Input::chain, chunk.rs:60-205; run.rs:67-83):
  * regular files are mapped read-only, "-" (or n_paths == 0) reads stdin and

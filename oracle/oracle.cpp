@@ -1465,3 +1465,161 @@ int64_t oracle_lines(const uint8_t *data, uint64_t n, uint64_t first_line, uint6
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// CSV input (no reference implementation: this build's own specification,
+// restated here independently of the device transcoder so the GPU tests
+// compare two implementations of it; parity unpinned)
+//
+// The first line is the header; its fields name the columns. Every later line
+// is one record, rendered as the NDJSON object {"name":value,...} in column
+// order, and the header itself as a blank line, so line numbers stay the
+// file's. A trailing CR is dropped from every line. A field is quoted when it
+// starts with '"'; inside, "" is one quote; the closing quote must end the
+// line or be followed by a comma. An unquoted field may not contain a quote.
+// Values: an empty unquoted field is null, an unquoted field that is a JSON
+// number is that number, anything else (every quoted field) is a string.
+// A line with bad quoting or a field count other than the header's becomes
+// the invalid line "{" (counted malformed); an empty line stays blank.
+// ---------------------------------------------------------------------------
+
+namespace csvo {
+
+// Fields of one line (without its newline or CR); false on bad quoting.
+static bool split(const uint8_t *b, size_t n, std::vector<std::pair<std::string, bool>> &f) {
+  f.clear();
+  size_t i = 0;
+  for (;;) {
+    std::string v;
+    bool quoted = false;
+    if (i < n && b[i] == '"') {
+      quoted = true;
+      size_t j = i + 1;
+      for (;;) {
+        const uint8_t *q = (const uint8_t *)memchr(b + j, '"', n - j);
+        if (!q) return false;
+        const size_t k = (size_t)(q - b);
+        v.append((const char *)b + j, k - j);
+        if (k + 1 < n && b[k + 1] == '"') {
+          v.push_back('"');
+          j = k + 2;
+          continue;
+        }
+        i = k + 1;
+        break;
+      }
+      if (i < n && b[i] != ',') return false;
+    } else {
+      size_t k = i;
+      while (k < n && b[k] != ',') {
+        if (b[k] == '"') return false;
+        k++;
+      }
+      v.assign((const char *)b + i, k - i);
+      i = k;
+    }
+    f.emplace_back(std::move(v), quoted);
+    if (i >= n) return true;
+    i++;  // the comma; a line ending in one has an empty last field
+  }
+}
+
+static bool json_number(const std::string &s) {
+  size_t i = 0, n = s.size();
+  if (i < n && s[i] == '-') i++;
+  if (i >= n) return false;
+  if (s[i] == '0') {
+    i++;
+  } else if (s[i] >= '1' && s[i] <= '9') {
+    while (i < n && s[i] >= '0' && s[i] <= '9') i++;
+  } else {
+    return false;
+  }
+  if (i < n && s[i] == '.') {
+    i++;
+    size_t d = i;
+    while (i < n && s[i] >= '0' && s[i] <= '9') i++;
+    if (i == d) return false;
+  }
+  if (i < n && (s[i] == 'e' || s[i] == 'E')) {
+    i++;
+    if (i < n && (s[i] == '+' || s[i] == '-')) i++;
+    size_t d = i;
+    while (i < n && s[i] >= '0' && s[i] <= '9') i++;
+    if (i == d) return false;
+  }
+  return i == n;
+}
+
+static void json_string(std::string &o, const std::string &s) {
+  static const char hex[] = "0123456789abcdef";
+  o.push_back('"');
+  for (unsigned char c : s) {
+    if (c == '"' || c == '\\') {
+      o.push_back('\\');
+      o.push_back((char)c);
+    } else if (c < 0x20) {
+      o += "\\u00";
+      o.push_back(hex[c >> 4]);
+      o.push_back(hex[c & 15]);
+    } else {
+      o.push_back((char)c);
+    }
+  }
+  o.push_back('"');
+}
+
+}  // namespace csvo
+
+extern "C" {
+
+// The NDJSON the CSV input reads as (see above); -1 when the header is bad.
+int oracle_csv_to_ndjson(const uint8_t *data, uint64_t n, uint8_t **out, uint64_t *out_len) {
+  *out = nullptr;
+  *out_len = 0;
+  std::string o;
+  std::vector<std::string> keys;
+  std::vector<std::pair<std::string, bool>> f;
+  uint64_t pos = 0;
+  bool header = true;
+  while (pos < n) {
+    const uint8_t *nl = (const uint8_t *)memchr(data + pos, '\n', n - pos);
+    const uint64_t end = nl ? (uint64_t)(nl - data) : n;
+    uint64_t len = end - pos;
+    if (len && data[pos + len - 1] == '\r') len--;
+    if (header) {
+      if (!csvo::split(data + pos, len, f)) return -1;
+      for (auto &x : f) {
+        std::string k;
+        csvo::json_string(k, x.first);
+        keys.push_back(k);
+      }
+      o.push_back('\n');
+      header = false;
+    } else if (len == 0) {
+      o.push_back('\n');
+    } else if (!csvo::split(data + pos, len, f) || f.size() != keys.size()) {
+      o += "{\n";
+    } else {
+      o.push_back('{');
+      for (size_t c = 0; c < f.size(); c++) {
+        if (c) o.push_back(',');
+        o += keys[c];
+        o.push_back(':');
+        const std::string &v = f[c].first;
+        if (!f[c].second && v.empty()) o += "null";
+        else if (!f[c].second && csvo::json_number(v)) o += v;
+        else csvo::json_string(o, v);
+      }
+      o += "}\n";
+    }
+    pos = nl ? end + 1 : n;
+  }
+  if (header) return -1;  // no header line
+  *out = (uint8_t *)malloc(o.size() ? o.size() : 1);
+  memcpy(*out, o.data(), o.size());
+  *out_len = o.size();
+  return 0;
+}
+
+}  // extern "C"

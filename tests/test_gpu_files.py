@@ -87,8 +87,8 @@ def test_pipe_streamed_with_long_lines(tmp_path):
     big = b'{"status":500,"bytes":7,"ua":"' + b"x" * 300000 + b'"}\n'
     data = BOM + corpus.nginx(9, 3000) + big + corpus.nginx(10, 50) + big + b'{"status":500}'
     stream = chained([data])
-    for q in ("select(.status == 500) | count", "select(.status == 500) | {b: .bytes}", "sum(.bytes)"):
-        fifo = Fifo(tmp_path, data, name="p" + str(abs(hash(q)) % 1000))
+    for i, q in enumerate(("select(.status == 500) | count", "select(.status == 500) | {b: .bytes}", "sum(.bytes)")):
+        fifo = Fifo(tmp_path, data, name="p%d" % i)
         gpu, st = w.run_paths(q, [fifo.path], chunk_bytes=1 << 16)
         fifo.join()
         cpu, _ = oracle.run(q, stream, streamed=True)

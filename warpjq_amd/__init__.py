@@ -103,7 +103,7 @@ REFERENCE_SYMBOLS = ["warpjq_probe", "warpjq_abi_check", "warpjq_ctx_create", "w
 ADDITIVE_SYMBOLS = ["warpjq_amd_ext_value", "warpjq_amd_wait_ext", "warpjq_amd_submit_device",
                     "warpjq_amd_slot_timing", "warpjq_amd_kernel_width", "warpjq_amd_set_claim", "warpjq_amd_set_arena",
                     "warpjq_amd_grid", "warpjq_amd_submit_spans", "warpjq_amd_fallback_spans",
-                    "warpjq_amd_group_keys", "warpjq_amd_submit_pieces", "warpjq_amd_run_buffer",
+                    "warpjq_amd_group_keys", "warpjq_amd_submit_pieces", "warpjq_amd_run_buffer", "warpjq_amd_run_csv",
                     "warpjq_amd_run_files", "warpjq_amd_device_upload", "warpjq_amd_device_free",
                     "warpjq_amd_server_create", "warpjq_amd_server_destroy", "warpjq_amd_server_add_dataset",
                     "warpjq_amd_server_submit", "warpjq_amd_server_wait", "warpjq_amd_server_get_stats",
@@ -164,6 +164,8 @@ def lib():
         L.warpjq_amd_run_buffer.argtypes = [cp, ctypes.c_void_p, u64, ctypes.c_int, ctypes.c_int, u64, u64,
                                             P(P(ctypes.c_uint8)), P(u64), P(Stats), cp, sz]
         L.warpjq_amd_run_buffer.restype = ctypes.c_int
+        L.warpjq_amd_run_csv.argtypes = L.warpjq_amd_run_buffer.argtypes
+        L.warpjq_amd_run_csv.restype = ctypes.c_int
         L.warpjq_amd_run_files.argtypes = [cp, P(ctypes.c_void_p), P(u64), u32, ctypes.c_int, ctypes.c_int, u64,
                                            u64, P(P(ctypes.c_uint8)), P(u64), P(Stats), cp, sz]
         L.warpjq_amd_run_files.restype = ctypes.c_int
@@ -314,6 +316,24 @@ def run(query, data, fmt=FMT_NDJSON, on_invalid=SKIP, chunk_bytes=GPU_CHUNK, max
     q = query.encode() if isinstance(query, str) else query
     rc = L.warpjq_amd_run_buffer(q, addr, nbytes, fmt, on_invalid, chunk_bytes, max_line, ctypes.byref(out),
                                  ctypes.byref(out_len), ctypes.byref(st), e, 2048)
+    del keep
+    return _finish(L, rc, out, out_len, st, e)
+
+
+def run_csv(query, data, fmt=FMT_NDJSON, on_invalid=SKIP, chunk_bytes=GPU_CHUNK, max_line=MAX_LINE):
+    """Runs `query` over a CSV input (header line, then one record per line,
+    transcoded to NDJSON objects on the device; warpjq_amd_run_csv).
+    Returns (output bytes, stats)."""
+    L = lib()
+    keep = data
+    addr, nbytes = _addr(data)
+    out = P(ctypes.c_uint8)()
+    out_len = u64()
+    st = Stats()
+    e = ctypes.create_string_buffer(2048)
+    q = query.encode() if isinstance(query, str) else query
+    rc = L.warpjq_amd_run_csv(q, addr, nbytes, fmt, on_invalid, chunk_bytes, max_line, ctypes.byref(out),
+                              ctypes.byref(out_len), ctypes.byref(st), e, 2048)
     del keep
     return _finish(L, rc, out, out_len, st, e)
 

@@ -39,6 +39,7 @@ const char *USAGE =
     "\n"
     "options:\n"
     "  --csv                  emit CSV instead of NDJSON\n"
+    "  --input-csv            read one CSV input (header line, then one record per line) instead of NDJSON\n"
     "  -c, --count            print only how many rows the query produced\n"
     "  --backend <auto|gpu|cpu>  engine (this build runs the GPU; cpu is refused)\n"
     "  --strict               stop at the first malformed line\n"
@@ -186,7 +187,7 @@ int run_gen(int argc, char **argv) {
 int main(int argc, char **argv) {
   if (argc >= 2 && !strcmp(argv[1], "gen")) return run_gen(argc, argv);
   std::string query;
-  bool have_query = false, csv = false, count = false, strict = false, skip = false, stats = false;
+  bool have_query = false, csv = false, count = false, strict = false, skip = false, stats = false, input_csv = false;
   std::string backend = "auto", chunk = "256MB", max_line = "64MB";
   std::vector<const char *> files;
   for (int i = 1; i < argc; i++) {
@@ -208,6 +209,8 @@ int main(int argc, char **argv) {
       return 0;
     } else if (key == "--csv") {
       csv = true;
+    } else if (key == "--input-csv") {
+      input_csv = true;
     } else if (key == "-c" || key == "--count") {
       count = true;
     } else if (key == "--strict") {
@@ -235,6 +238,7 @@ int main(int argc, char **argv) {
   }
   if (!have_query) return usage_error("no query given");
   if (csv && count) return usage_error("the argument '--csv' cannot be used with '--count'");
+  if (input_csv && files.size() > 1) return usage_error("--input-csv reads one input");
   if (strict && skip) return usage_error("the argument '--strict' cannot be used with '--skip-invalid'");
   if (backend != "auto" && backend != "gpu" && backend != "cpu")
     return usage_error("invalid value '" + backend + "' for '--backend' (possible values: auto, gpu, cpu)");
@@ -271,8 +275,25 @@ int main(int argc, char **argv) {
   uint64_t out_len = 0;
   warpjq_amd_stats st;
   const auto t0 = std::chrono::steady_clock::now();
-  const int rc = warpjq_amd_run_paths(query.c_str(), files.data(), (uint32_t)files.size(), format, on_invalid,
-                                      chunk_bytes, max_line_bytes, &out, &out_len, &st, err, sizeof err);
+  int rc;
+  if (input_csv) {
+    // the whole input, then the device transcodes its records to NDJSON
+    std::vector<uint8_t> buf;
+    FILE *f = files.empty() || strcmp(files[0], "-") == 0 ? stdin : fopen(files[0], "rb");
+    if (!f) {
+      fprintf(stderr, "warpjq-amd: could not open %s: %s\n", files[0], strerror(errno));
+      return 1;
+    }
+    uint8_t tmp[1 << 16];
+    size_t got;
+    while ((got = fread(tmp, 1, sizeof tmp, f)) > 0) buf.insert(buf.end(), tmp, tmp + got);
+    if (f != stdin) fclose(f);
+    rc = warpjq_amd_run_csv(query.c_str(), buf.data(), buf.size(), format, on_invalid, chunk_bytes, max_line_bytes,
+                            &out, &out_len, &st, err, sizeof err);
+  } else {
+    rc = warpjq_amd_run_paths(query.c_str(), files.data(), (uint32_t)files.size(), format, on_invalid, chunk_bytes,
+                              max_line_bytes, &out, &out_len, &st, err, sizeof err);
+  }
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   const std::string name = files.empty() ? "<stdin>" : files.size() == 1 ? files[0]
                                                         : std::string(files[0]) + " (+" +

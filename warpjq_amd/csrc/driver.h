@@ -114,6 +114,12 @@ int engine_run(Engine &g, Source &src, int format, int on_invalid, uint8_t **out
 // memory. Row outputs over a span inside a registered dataset count each
 // claim's lines from the index instead of reading the whole claim.
 uint64_t nlindex_bytes(uint64_t n);
+
+// CSV input (csv.hip): the NDJSON the records of data[0..n) read as, built
+// on the device (the header as a blank line); *records = lines after the
+// header; *kernel_ms = the transcoding kernels' time.
+bool csv_transcode(const uint8_t *data, uint64_t n, std::vector<uint8_t> &out, uint64_t *records, float *kernel_ms,
+                   std::string &err);
 // Builds the index of d[0..n) into idx (device memory, nlindex_bytes(n)
 // bytes); waits for it.
 bool nlindex_build(const uint8_t *d, uint64_t n, uint8_t *idx, std::string &err);

@@ -1174,6 +1174,29 @@ int warpjq_amd_run_buffer(const char *query, const uint8_t *data, uint64_t n, in
   return run_once(query, src, format, on_invalid, chunk_bytes, out, out_len, stats, err, err_len);
 }
 
+int warpjq_amd_run_csv(const char *query, const uint8_t *data, uint64_t n, int format, int on_invalid,
+                       uint64_t chunk_bytes, uint64_t max_line_bytes, uint8_t **out, uint64_t *out_len,
+                       warpjq_amd_stats *stats, char *err, size_t err_len) {
+  std::vector<uint8_t> nd;
+  uint64_t records = 0;
+  float kms = 0;
+  std::string e;
+  if (out) *out = nullptr;
+  if (out_len) *out_len = 0;
+  if (stats) memset(stats, 0, sizeof *stats);
+  if (!wjqd::csv_transcode(data, n, nd, &records, &kms, e)) {
+    if (err && err_len) snprintf(err, err_len, "%s", e.c_str());
+    return 3;
+  }
+  const int rc = warpjq_amd_run_buffer(query, nd.data(), nd.size(), format, on_invalid, chunk_bytes, max_line_bytes, out,
+                                       out_len, stats, err, err_len);
+  if (stats) {
+    stats->bytes_in = n;
+    stats->kernel_ms += kms;
+  }
+  return rc;
+}
+
 int warpjq_amd_run_files(const char *query, const uint8_t *const *files, const uint64_t *lens, uint32_t n_files,
                          int format, int on_invalid, uint64_t chunk_bytes, uint64_t max_line_bytes, uint8_t **out,
                          uint64_t *out_len, warpjq_amd_stats *stats, char *err, size_t err_len) {
