@@ -19,7 +19,11 @@
 // Device passes: newlines per 4 KiB tile, an exclusive scan, every newline's
 // position; then per record its output length, a scan, and the output bytes.
 // One lane walks one record (records are short and independent).
+#include <sys/mman.h>
+
 #include <hipcub/hipcub.hpp>
+#include <mutex>
+#include <thread>
 
 #include "driver.h"
 #include "kernels.h"
@@ -27,6 +31,7 @@
 namespace {
 
 constexpr uint32_t CSV_TILE = 4096;  // bytes per wave in the newline passes
+constexpr uint64_t CSV_DEV_PAD = 256;  // readable bytes the kernels may touch past a span
 
 struct CsvHdr {
   uint32_t ncols;
@@ -350,13 +355,103 @@ bool excl_scan(T *in_out, uint64_t n, hipStream_t st, CsvBufs &B, std::string &e
   return true;
 }
 
+// Pinned staging for the copies in and out: two 32 MiB buffers, taken on
+// first use and kept for the life of the process. One transcoding at a time
+// holds them (the mutex).
+constexpr uint64_t STAGE = 32ull << 20;
+
+struct Staging {
+  std::mutex mu;
+  uint8_t *buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+Staging &staging() {
+  static Staging s;
+  return s;
+}
+
+bool staging_ready(Staging &S, std::string &err) {
+  for (int b = 0; b < 2; b++) {
+    if (!S.buf[b]) CSV_TRY(hipHostMalloc((void **)&S.buf[b], STAGE, hipHostMallocDefault), "hipHostMalloc(stage)");
+    if (!S.ev[b]) CSV_TRY(hipEventCreateWithFlags(&S.ev[b], hipEventDisableTiming), "hipEventCreate(stage)");
+  }
+  return true;
+}
+
+// memcpy split over up to 8 threads (the caller's included), 4 KiB-aligned
+// parts, so that first-touch page faults of a fresh destination run in
+// parallel too.
+void par_copy(uint8_t *dst, const uint8_t *src, uint64_t n) {
+  const uint64_t want = std::min<uint64_t>(8, std::max<uint64_t>(1, n >> 22));
+  const unsigned T = (unsigned)std::min<uint64_t>(want, std::max(1u, std::thread::hardware_concurrency()));
+  if (T <= 1) {
+    memcpy(dst, src, n);
+    return;
+  }
+  const uint64_t per = ((n + T - 1) / T + 4095) & ~4095ull;
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < T; t++) {
+    const uint64_t a = std::min(n, per * t), b = std::min(n, per * (t + 1));
+    if (a < b) th.emplace_back([=] { memcpy(dst + a, src + a, b - a); });
+  }
+  memcpy(dst, src, std::min(n, per));
+  for (std::thread &x : th) x.join();
+}
+
+// host h[0..n) -> device d: each 32 MiB stage copied into one pinned buffer
+// while the DMA reads the other
+bool h2d_staged(Staging &S, uint8_t *d, const uint8_t *h, uint64_t n, hipStream_t st, std::string &err) {
+  for (uint64_t off = 0, k = 0; off < n; off += STAGE, k++) {
+    const int b = (int)(k & 1);
+    const uint64_t len = std::min(STAGE, n - off);
+    CSV_TRY(hipEventSynchronize(S.ev[b]), "sync(stage)");  // the DMA that last read buf[b] is done
+    par_copy(S.buf[b], h + off, len);
+    CSV_TRY(hipMemcpyAsync(d + off, S.buf[b], len, hipMemcpyHostToDevice, st), "H2D(csv)");
+    CSV_TRY(hipEventRecord(S.ev[b], st), "hipEventRecord(stage)");
+  }
+  return true;
+}
+
+// device d[0..n) -> host h: stage k+1's DMA runs while stage k is copied out
+bool d2h_staged(Staging &S, uint8_t *h, const uint8_t *d, uint64_t n, hipStream_t st, std::string &err) {
+  const uint64_t nst = (n + STAGE - 1) / STAGE;
+  auto issue = [&](uint64_t k) -> hipError_t {
+    const uint64_t off = k * STAGE;
+    hipError_t e = hipMemcpyAsync(S.buf[k & 1], d + off, std::min(STAGE, n - off), hipMemcpyDeviceToHost, st);
+    return e == hipSuccess ? hipEventRecord(S.ev[k & 1], st) : e;
+  };
+  if (nst) CSV_TRY(issue(0), "D2H(ndjson)");
+  for (uint64_t k = 0; k < nst; k++) {
+    // buf[(k + 1) & 1] was last copied out in step k - 1, which has returned
+    if (k + 1 < nst) CSV_TRY(issue(k + 1), "D2H(ndjson)");
+    CSV_TRY(hipEventSynchronize(S.ev[k & 1]), "sync(stage)");
+    const uint64_t off = k * STAGE;
+    par_copy(h + off, S.buf[k & 1], std::min(STAGE, n - off));
+  }
+  return true;
+}
+
+// Uninitialised host memory on 2 MiB boundaries, advised for huge pages
+// (fewer first-touch faults); freed with free().
+uint8_t *host_alloc(uint64_t n) {
+  const uint64_t al = 2ull << 20, sz = std::max<uint64_t>(al, (n + al - 1) & ~(al - 1));
+  void *p = nullptr;
+  if (posix_memalign(&p, al, sz)) return nullptr;
+  (void)madvise(p, sz, MADV_HUGEPAGE);
+  return (uint8_t *)p;
+}
+
 }  // namespace
 
 namespace wjqd {
 
-bool csv_transcode(const uint8_t *data, uint64_t n, std::vector<uint8_t> &out, uint64_t *records, float *kernel_ms,
-                   std::string &err) {
-  out.clear();
+bool csv_transcode(const uint8_t *data, uint64_t n, HostBytes &out, uint8_t **dev_out, uint64_t *records,
+                   float *kernel_ms, std::string &err) {
+  free(out.p);
+  out.p = nullptr;
+  out.n = 0;
+  *dev_out = nullptr;
   *records = 0;
   if (kernel_ms) *kernel_ms = 0;
   const uint8_t *nl = (const uint8_t *)memchr(data, '\n', n);
@@ -379,27 +474,47 @@ bool csv_transcode(const uint8_t *data, uint64_t n, std::vector<uint8_t> &out, u
   }
   // the body, ending in a newline, zero-padded to whole tiles
   const uint64_t b0 = nl ? hend + 1 : n;
-  uint64_t body = n - b0;
+  const uint64_t body = n - b0;
   const bool add_nl = body && data[n - 1] != '\n';
   const uint64_t blen = body + (add_nl ? 1 : 0);
   const uint64_t ntiles = (blen + CSV_TILE - 1) / CSV_TILE;
-  out.assign(1, '\n');  // the header's line
-  if (blen == 0) return true;
   if (ntiles > 0xFFFFFFFFull || blen > (1ull << 40)) {
     err = "CSV input: too large";
     return false;
   }
+  if (blen == 0) {  // header only: one blank line
+    out.p = host_alloc(1);
+    uint8_t *o = nullptr;
+    if (!out.p) {
+      err = "CSV input: out of host memory";
+      return false;
+    }
+    out.p[0] = '\n';
+    out.n = 1;
+    CSV_TRY(hipMalloc((void **)&o, 1 + CSV_DEV_PAD), "hipMalloc(ndjson)");
+    *dev_out = o;  // the caller's from here on
+    CSV_TRY(hipMemcpy(o, out.p, 1, hipMemcpyHostToDevice), "H2D(ndjson)");
+    return true;
+  }
+  Staging &S = staging();
+  std::lock_guard<std::mutex> hold(S.mu);
+  if (!staging_ready(S, err)) return false;
   CsvBufs B;
   hipStream_t st = nullptr;
   CSV_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
   struct StreamGuard {
     hipStream_t s;
-    ~StreamGuard() { (void)hipStreamDestroy(s); }
+    ~StreamGuard() {
+      (void)hipStreamSynchronize(s);  // no staged copy outlives the call
+      (void)hipStreamDestroy(s);
+    }
   } sg{st};
+  CSV_TRY(hipEventRecord(S.ev[0], st), "hipEventRecord(stage)");
+  CSV_TRY(hipEventRecord(S.ev[1], st), "hipEventRecord(stage)");
   uint8_t *d = nullptr;
   CSV_TRY(B.get(&d, ntiles * CSV_TILE + 64), "hipMalloc(csv)");
   CSV_TRY(hipMemsetAsync(d + blen, 0, ntiles * CSV_TILE + 64 - blen, st), "memset(csv)");
-  CSV_TRY(hipMemcpyAsync(d, data + b0, body, hipMemcpyHostToDevice, st), "H2D(csv)");
+  if (!h2d_staged(S, d, data + b0, body, st, err)) return false;
   if (add_nl) CSV_TRY(hipMemsetAsync(d + body, '\n', 1, st), "memset(csv)");
   uint8_t *dk = nullptr;
   uint32_t *dko = nullptr, *dkl = nullptr;
@@ -459,17 +574,30 @@ bool csv_transcode(const uint8_t *data, uint64_t n, std::vector<uint8_t> &out, u
     err = "CSV input: a record of 4 GiB or more";
     return false;
   }
+  // o[0] is the header's blank line, the records follow; the device copy
+  // keeps the pad the kernels may read past a span
   uint8_t *o = nullptr;
-  CSV_TRY(B.get(&o, total), "hipMalloc(ndjson)");
-  hipLaunchKernelGGL(k_csv_emit, dim3(std::max(1u, g2)), dim3(256), 0, st, d, pos, nrec, h, len, o);
+  CSV_TRY(hipMalloc((void **)&o, 1 + total + CSV_DEV_PAD), "hipMalloc(ndjson)");
+  *dev_out = o;  // the caller's from here on, also on an error below
+  CSV_TRY(hipMemsetAsync(o, '\n', 1, st), "memset(ndjson)");
+  hipLaunchKernelGGL(k_csv_emit, dim3(std::max(1u, g2)), dim3(256), 0, st, d, pos, nrec, h, len, o + 1);
   CSV_TRY(hipGetLastError(), "k_csv_emit");
   CSV_TRY(hipEventRecord(e1, st), "hipEventRecord");
-  out.resize(1 + total);
-  CSV_TRY(hipMemcpyAsync(out.data() + 1, o, total, hipMemcpyDeviceToHost, st), "D2H(ndjson)");
+  out.p = host_alloc(1 + total);
+  if (!out.p) {
+    err = "CSV input: out of host memory";
+    return false;
+  }
+  out.n = 1 + total;
+  if (!d2h_staged(S, out.p, o, 1 + total, st, err)) return false;
   CSV_TRY(hipStreamSynchronize(st), "sync");
   if (kernel_ms) (void)hipEventElapsedTime(kernel_ms, e0, e1);
   *records = nrec;
   return true;
+}
+
+void csv_free(uint8_t *dev) {
+  if (dev) (void)hipFree(dev);
 }
 
 }  // namespace wjqd

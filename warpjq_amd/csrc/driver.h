@@ -2,6 +2,7 @@
 // serving loop (serve.cpp). Not exported.
 #pragma once
 #include <cstdint>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -115,11 +116,25 @@ int engine_run(Engine &g, Source &src, int format, int on_invalid, uint8_t **out
 // claim's lines from the index instead of reading the whole claim.
 uint64_t nlindex_bytes(uint64_t n);
 
+// Host bytes from malloc, freed with the object.
+struct HostBytes {
+  uint8_t *p = nullptr;
+  uint64_t n = 0;
+  HostBytes() = default;
+  HostBytes(const HostBytes &) = delete;
+  HostBytes &operator=(const HostBytes &) = delete;
+  ~HostBytes() { free(p); }
+};
+
 // CSV input (csv.hip): the NDJSON the records of data[0..n) read as, built
-// on the device (the header as a blank line); *records = lines after the
-// header; *kernel_ms = the transcoding kernels' time.
-bool csv_transcode(const uint8_t *data, uint64_t n, std::vector<uint8_t> &out, uint64_t *records, float *kernel_ms,
-                   std::string &err);
+// on the device (the header as a blank line), in device memory (*dev_out,
+// 256 readable bytes past its end) and copied to the host (out);
+// *records = lines after the header; *kernel_ms = the transcoding kernels'
+// time. The caller frees *dev_out with csv_free, also when false is
+// returned. Calls from several threads take turns.
+bool csv_transcode(const uint8_t *data, uint64_t n, HostBytes &out, uint8_t **dev_out, uint64_t *records,
+                   float *kernel_ms, std::string &err);
+void csv_free(uint8_t *dev);
 // Builds the index of d[0..n) into idx (device memory, nlindex_bytes(n)
 // bytes); waits for it.
 bool nlindex_build(const uint8_t *d, uint64_t n, uint8_t *idx, std::string &err);

@@ -1177,23 +1177,34 @@ int warpjq_amd_run_buffer(const char *query, const uint8_t *data, uint64_t n, in
 int warpjq_amd_run_csv(const char *query, const uint8_t *data, uint64_t n, int format, int on_invalid,
                        uint64_t chunk_bytes, uint64_t max_line_bytes, uint8_t **out, uint64_t *out_len,
                        warpjq_amd_stats *stats, char *err, size_t err_len) {
-  std::vector<uint8_t> nd;
+  wjqd::HostBytes nd;
   uint64_t records = 0;
   float kms = 0;
   std::string e;
-  if (out) *out = nullptr;
-  if (out_len) *out_len = 0;
-  if (stats) memset(stats, 0, sizeof *stats);
-  if (!wjqd::csv_transcode(data, n, nd, &records, &kms, e)) {
-    if (err && err_len) snprintf(err, err_len, "%s", e.c_str());
+  uint8_t *dev = nullptr;
+  *out = nullptr;
+  *out_len = 0;
+  memset(stats, 0, sizeof *stats);
+  if (!data && n) {
+    snprintf(err, err_len, "run_csv: null input");
     return 3;
   }
-  const int rc = warpjq_amd_run_buffer(query, nd.data(), nd.size(), format, on_invalid, chunk_bytes, max_line_bytes, out,
-                                       out_len, stats, err, err_len);
-  if (stats) {
-    stats->bytes_in = n;
-    stats->kernel_ms += kms;
+  if (!wjqd::csv_transcode(data, n, nd, &dev, &records, &kms, e)) {
+    wjqd::csv_free(dev);
+    snprintf(err, err_len, "%s", e.c_str());
+    return 3;
   }
+  // the run reads the transcoded bytes where they already are, in device
+  // memory; the host copy serves overflow splits and row cuts
+  const uint8_t *hp = nd.p;
+  uint64_t hn = nd.n;
+  const uint8_t *dp = dev;
+  wjqd::Batcher src(&hp, &hn, 1, max_line_bytes, 1);
+  src.dfiles = &dp;
+  const int rc = run_once(query, src, format, on_invalid, chunk_bytes, out, out_len, stats, err, err_len);
+  wjqd::csv_free(dev);
+  stats->bytes_in = n;
+  stats->kernel_ms += kms;
   return rc;
 }
 
