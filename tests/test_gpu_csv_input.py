@@ -139,6 +139,31 @@ def test_csv_header_errors():
     assert w.run_csv("count", b"a,b")[0] == b"0\n"
 
 
+def test_csv_long_and_expanding_records():
+    # groups of 64 records whose bytes pass the LDS stage (long quoted
+    # fields), whose output does (control bytes written as \u00XX), and
+    # ordinary ones around them
+    rng = random.Random(21)
+    lines = [b"id,blob,n"]
+    for i in range(4000):
+        r = rng.random()
+        if r < 0.01:
+            blob = b'"' + b"x" * rng.randrange(16000, 40000) + b'"'
+        elif r < 0.02:
+            blob = b'"' + b"\t\x01" * rng.randrange(1000, 3000) + b'"'
+        elif r < 0.03:
+            blob = b'"' + b'""' * rng.randrange(100, 4000) + b'"'
+        else:
+            blob = rng.choice([b"a", b'"b,c"', b"", b"12"])
+        lines.append(b"%d,%s,%d" % (i, blob, rng.randrange(1000)))
+    data = b"\n".join(lines) + b"\n"
+    for q in (".", "count", "select(.n > 500) | {id: .id, b: .blob}", "group_by(.blob) | count"):
+        gpu, gst = w.run_csv(q, data)
+        cpu, cst = oracle.run_csv(q, data, threads=4)
+        assert gpu == cpu, q
+        assert gst["lines_in"] == cst["lines_in"]
+
+
 def test_csv_input_at_size():
     # ~9 MiB: many newline tiles and records per transcoding pass
     data = csv_corpus(11, 60000)

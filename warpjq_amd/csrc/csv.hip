@@ -18,7 +18,8 @@
 //     blank.
 // Device passes: newlines per 4 KiB tile, an exclusive scan, every newline's
 // position; then per record its output length, a scan, and the output bytes.
-// One lane walks one record (records are short and independent).
+// One lane walks one record (records are short and independent), from LDS
+// where its group of 64 records fits there (k_csv_pass).
 #include <sys/mman.h>
 
 #include <hipcub/hipcub.hpp>
@@ -78,7 +79,7 @@ __global__ __launch_bounds__(256) void k_csv_nl_pos(const uint8_t *d, uint64_t n
 __device__ __forceinline__ bool csv_digit(uint32_t c) { return c - '0' < 10u; }
 
 // [a, b) of the record is a JSON number.
-__device__ bool csv_number(const uint8_t *r, uint32_t a, uint32_t b) {
+__device__ __forceinline__ bool csv_number(const uint8_t *r, uint32_t a, uint32_t b) {
   uint32_t i = a;
   if (i < b && r[i] == '-') i++;
   if (i >= b) return false;
@@ -105,7 +106,7 @@ __device__ bool csv_number(const uint8_t *r, uint32_t a, uint32_t b) {
 }
 
 // The fields of record r[0..n): false on bad quoting, else their count.
-__device__ bool csv_count(const uint8_t *r, uint32_t n, uint32_t &fields) {
+__device__ __forceinline__ bool csv_count(const uint8_t *r, uint32_t n, uint32_t &fields) {
   uint32_t i = 0;
   fields = 0;
   for (;;) {
@@ -150,7 +151,7 @@ struct ByteSink {
 };
 
 template <class S>
-__device__ void csv_escaped(S &s, uint32_t c) {
+__device__ __forceinline__ void csv_escaped(S &s, uint32_t c) {
   if (c == '"' || c == '\\') {
     s.put('\\');
     s.put((uint8_t)c);
@@ -170,7 +171,7 @@ __device__ void csv_escaped(S &s, uint32_t c) {
 // The NDJSON line (with its newline) of record r[0..n), already counted and
 // quoted correctly (csv_count).
 template <class S>
-__device__ void csv_object(const uint8_t *r, uint32_t n, const CsvHdr &h, S &s) {
+__device__ __forceinline__ void csv_object(const uint8_t *r, uint32_t n, const CsvHdr &h, S &s) {
   s.put('{');
   uint32_t i = 0;
   for (uint32_t c = 0;; c++) {
@@ -229,7 +230,7 @@ __device__ __forceinline__ const uint8_t *csv_record(const uint8_t *d, const uin
 }
 
 template <class S>
-__device__ void csv_line(const uint8_t *r, uint32_t n, const CsvHdr &h, S &s) {
+__device__ __forceinline__ void csv_line(const uint8_t *r, uint32_t n, const CsvHdr &h, S &s) {
   uint32_t f;
   if (n == 0) {
     s.put('\n');
@@ -241,27 +242,82 @@ __device__ void csv_line(const uint8_t *r, uint32_t n, const CsvHdr &h, S &s) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_csv_len(const uint8_t *d, const uint64_t *pos, uint64_t nrec, CsvHdr h,
-                                                  uint64_t *len, uint32_t *too_long) {
-  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nrec; k += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t n;
-    bool ok;
-    const uint8_t *r = csv_record(d, pos, k, n, ok);
-    if (!ok) *too_long = 1u;
-    CountSink s;
-    csv_line(r, n, h, s);
-    len[k] = s.n;
-  }
-}
+// Records go in groups of 64, one wave per group. A group whose bytes fit
+// in LDS is staged there with dword loads; each lane reads its record from
+// LDS and (EMIT) writes its object into LDS, and the wave stores the
+// group's output with dword stores, the ragged first and last bytes singly
+// (the neighbouring groups own the rest of those dwords). A larger group is
+// walked by each lane from global memory.
+constexpr uint32_t CSV_IN_LDS = 16384, CSV_OUT_LDS = 24576;
 
-__global__ __launch_bounds__(256) void k_csv_emit(const uint8_t *d, const uint64_t *pos, uint64_t nrec, CsvHdr h,
-                                                   const uint64_t *off, uint8_t *out) {
-  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nrec; k += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t n;
-    bool ok;
-    const uint8_t *r = csv_record(d, pos, k, n, ok);
-    ByteSink s{out + off[k]};
-    csv_line(r, n, h, s);
+// EMIT false: len[k] = the length of record k's line (too_long set for a
+// record of 4 GiB or more). EMIT true: the line of record k written at
+// out[shift + off[k]].
+template <bool EMIT>
+__global__ __launch_bounds__(64) void k_csv_pass(const uint8_t *d, const uint64_t *pos, uint64_t nrec, CsvHdr h,
+                                                 uint64_t *len, uint32_t *too_long, const uint64_t *off, uint8_t *out,
+                                                 uint64_t shift) {
+  __shared__ __align__(16) uint8_t lin[CSV_IN_LDS];
+  __shared__ __align__(16) uint8_t lout[EMIT ? CSV_OUT_LDS : 4];
+  const uint32_t lane = threadIdx.x;
+  const uint64_t ngroups = (nrec + 63) / 64;
+  for (uint64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    const uint64_t k0 = g * 64, k1 = min(nrec, k0 + 64), k = k0 + lane;
+    const uint64_t a = k0 ? pos[k0 - 1] + 1 : 0, b = pos[k1 - 1];
+    const uint64_t a4 = a & ~3ull, in_n = b + 1 - a4;
+    uint64_t o0 = 0, o1 = 0, o4 = 0;
+    if (EMIT) {
+      o0 = shift + off[k0];
+      o1 = shift + off[k1];
+      o4 = o0 & ~3ull;
+    }
+    if (in_n <= CSV_IN_LDS && (!EMIT || o1 - o4 <= CSV_OUT_LDS)) {
+      // dwords from a4 through the one holding b: at most 3 bytes past b,
+      // inside the input's zero pad
+      const uint32_t nw = (uint32_t)((in_n + 3) >> 2);
+      for (uint32_t i = lane; i < nw; i += 64) ((uint32_t *)lin)[i] = *(const uint32_t *)(d + a4 + 4ull * i);
+      __syncthreads();
+      if (k < k1) {
+        const uint64_t ra = k ? pos[k - 1] + 1 : 0, rb = pos[k];
+        uint32_t n = (uint32_t)(rb - ra);
+        if (n && lin[rb - 1 - a4] == '\r') n--;
+        const uint8_t *r = lin + (ra - a4);
+        if (EMIT) {
+          ByteSink s{lout + (shift + off[k] - o4)};
+          csv_line(r, n, h, s);
+        } else {
+          CountSink s;
+          csv_line(r, n, h, s);
+          len[k] = s.n;
+        }
+      }
+      __syncthreads();
+      if (EMIT) {
+        const uint64_t m0 = (o0 + 3) & ~3ull, m1 = o1 & ~3ull;
+        if (m0 >= m1) {
+          for (uint64_t i = o0 + lane; i < o1; i += 64) out[i] = lout[i - o4];
+        } else {
+          if (lane < m0 - o0) out[o0 + lane] = lout[o0 + lane - o4];
+          if (lane < o1 - m1) out[m1 + lane] = lout[m1 + lane - o4];
+          for (uint64_t i = m0 + 4ull * lane; i < m1; i += 256)
+            *(uint32_t *)(out + i) = *(const uint32_t *)(lout + (i - o4));
+        }
+        __syncthreads();  // the next group overwrites lout
+      }
+    } else if (k < k1) {
+      uint32_t n;
+      bool ok;
+      const uint8_t *r = csv_record(d, pos, k, n, ok);
+      if (EMIT) {
+        ByteSink s{out + shift + off[k]};
+        csv_line(r, n, h, s);
+      } else {
+        if (!ok) *too_long = 1u;
+        CountSink s;
+        csv_line(r, n, h, s);
+        len[k] = s.n;
+      }
+    }
   }
 }
 
@@ -560,9 +616,10 @@ bool csv_transcode(const uint8_t *data, uint64_t n, HostBytes &out, uint8_t **de
   hipLaunchKernelGGL(k_csv_nl_pos, dim3(g1), dim3(256), 0, st, d, ntiles, cnt, pos);
   CSV_TRY(hipGetLastError(), "k_csv_nl_pos");
   const CsvHdr h{(uint32_t)cols.size(), dk, dko, dkl};
-  const uint32_t g2 = (uint32_t)std::min<uint64_t>(8192, (nrec + 255) / 256);
-  hipLaunchKernelGGL(k_csv_len, dim3(std::max(1u, g2)), dim3(256), 0, st, d, pos, nrec, h, len, too_long);
-  CSV_TRY(hipGetLastError(), "k_csv_len");
+  const uint32_t g2 = (uint32_t)std::min<uint64_t>(8192, (nrec + 63) / 64);
+  hipLaunchKernelGGL(k_csv_pass<false>, dim3(g2), dim3(64), 0, st, d, pos, nrec, h, len, too_long, nullptr, nullptr,
+                     0ull);
+  CSV_TRY(hipGetLastError(), "k_csv_pass<len>");
   CSV_TRY(hipMemsetAsync(len + nrec, 0, 8, st), "memset(lengths)");
   if (!excl_scan(len, nrec + 1, st, B, err)) return false;
   uint64_t total = 0;
@@ -580,8 +637,8 @@ bool csv_transcode(const uint8_t *data, uint64_t n, HostBytes &out, uint8_t **de
   CSV_TRY(hipMalloc((void **)&o, 1 + total + CSV_DEV_PAD), "hipMalloc(ndjson)");
   *dev_out = o;  // the caller's from here on, also on an error below
   CSV_TRY(hipMemsetAsync(o, '\n', 1, st), "memset(ndjson)");
-  hipLaunchKernelGGL(k_csv_emit, dim3(std::max(1u, g2)), dim3(256), 0, st, d, pos, nrec, h, len, o + 1);
-  CSV_TRY(hipGetLastError(), "k_csv_emit");
+  hipLaunchKernelGGL(k_csv_pass<true>, dim3(g2), dim3(64), 0, st, d, pos, nrec, h, nullptr, nullptr, len, o, 1ull);
+  CSV_TRY(hipGetLastError(), "k_csv_pass<emit>");
   CSV_TRY(hipEventRecord(e1, st), "hipEventRecord");
   out.p = host_alloc(1 + total);
   if (!out.p) {
