@@ -1471,7 +1471,9 @@ int64_t oracle_lines(const uint8_t *data, uint64_t n, uint64_t first_line, uint6
 // restated here independently of the device transcoder so the GPU tests
 // compare two implementations of it; parity unpinned)
 //
-// The first line is the header; its fields name the columns. Every later line
+// The first line is the header (after a UTF-8 byte order mark, which is
+// dropped, as the NDJSON reader drops one per input); its fields name the
+// columns. Every later line
 // is one record, rendered as the NDJSON object {"name":value,...} in column
 // order, and the header itself as a blank line, so line numbers stay the
 // file's. A trailing CR is dropped from every line. A field is quoted when it
@@ -1580,7 +1582,8 @@ int oracle_csv_to_ndjson(const uint8_t *data, uint64_t n, uint8_t **out, uint64_
   std::string o;
   std::vector<std::string> keys;
   std::vector<std::pair<std::string, bool>> f;
-  uint64_t pos = 0;
+  uint64_t pos = (n >= 3 && data[0] == 0xEF && data[1] == 0xBB && data[2] == 0xBF) ? 3 : 0;
+  if (pos >= n) return -1;  // no header line
   bool header = true;
   while (pos < n) {
     const uint8_t *nl = (const uint8_t *)memchr(data + pos, '\n', n - pos);

@@ -131,9 +131,18 @@ def test_csv_strict_line_numbers_are_the_files():
     assert w.run_csv("count", data)[0] == oracle.run_csv("count", data)[0] == b"3\n"
 
 
+def test_csv_byte_order_mark():
+    data = b"\xef\xbb\xbfhost,status\na,500\nb,200\n"
+    for q in ("select(.status == 500) | .host", "."):
+        assert w.run_csv(q, data)[0] == oracle.run_csv(q, data)[0]
+    assert w.run_csv("select(.status == 500) | .host", data)[0] == b'"a"\n'
+
+
 def test_csv_header_errors():
     with pytest.raises(w.RunError):
         w.run_csv("count", b"")
+    with pytest.raises(w.RunError):
+        w.run_csv("count", b"\xef\xbb\xbf")
     with pytest.raises(w.RunError):
         w.run_csv("count", b'"open\n1\n')
     assert w.run_csv("count", b"a,b")[0] == b"0\n"

@@ -27,6 +27,8 @@ CASES = [
     (b'"k""q",k\\,k\n1,2,3\n', b'\n{"k\\"q":1,"k\\\\":2,"k":3}\n'),
     # true / false / null as text stay strings
     (b"v\ntrue\nnull\n", b'\n{"v":"true"}\n{"v":"null"}\n'),
+    # a leading UTF-8 byte order mark is dropped; one elsewhere is data
+    (b"\xef\xbb\xbfa,b\n1,\xef\xbb\xbf\n", b'\n{"a":1,"b":"\xef\xbb\xbf"}\n'),
 ]
 
 
@@ -55,5 +57,7 @@ def test_csv_queries_read_the_records():
 def test_csv_without_header_is_an_error():
     with pytest.raises(ValueError):
         oracle.csv_to_ndjson(b"")
+    with pytest.raises(ValueError):
+        oracle.csv_to_ndjson(b"\xef\xbb\xbf")
     with pytest.raises(ValueError):
         oracle.csv_to_ndjson(b'"open\n1\n')

@@ -3,8 +3,9 @@
 // The reference reads NDJSON only; CSV input is this build's own format, so
 // its results are pinned only against the oracle's independent restatement
 // of the same specification (oracle/oracle.cpp, "CSV input"):
-//   * the first line is the header and names the columns; it becomes a blank
-//     line, so line numbers stay the file's;
+//   * the first line is the header (after a UTF-8 byte order mark, which is
+//     dropped, as the NDJSON reader drops one per input) and names the
+//     columns; it becomes a blank line, so line numbers stay the file's;
 //   * every later line is one record, the object {"name":value,...} in
 //     column order; a trailing CR is dropped;
 //   * a field is quoted when it starts with '"'; inside, "" is one quote; the
@@ -510,6 +511,10 @@ bool csv_transcode(const uint8_t *data, uint64_t n, HostBytes &out, uint8_t **de
   *dev_out = nullptr;
   *records = 0;
   if (kernel_ms) *kernel_ms = 0;
+  if (n >= 3 && data[0] == 0xEF && data[1] == 0xBB && data[2] == 0xBF) {  // a byte order mark
+    data += 3;
+    n -= 3;
+  }
   const uint8_t *nl = (const uint8_t *)memchr(data, '\n', n);
   const uint64_t hend = nl ? (uint64_t)(nl - data) : n;
   uint64_t hlen = hend;
