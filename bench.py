@@ -5,7 +5,9 @@ nginx-schema NDJSON already resident in HBM (the chunk split in <=4 GiB
 pieces, each through warpjq_amd_submit_device). `value` is input GB/s over
 the timed steps. Also reported, once each after the timed loop: the other
 three queries of the same workload (kernel-only GB/s), the end-to-end
-host-buffer run (pinned H2D + kernels), the roofline of the main kernel
+host-buffer run (pinned H2D + kernels), the batched form on the first 1 GiB
+cut into 1,024 files (one launch each against one launch for all), the
+roofline of the main kernel
 (hipEvents around it, on its own stream) and the CPU baseline (the oracle on
 a bounded sample, host threads).
 
@@ -212,6 +214,64 @@ def bench_files(args, w, torch, np, oracle):
         "csv_input": csv_in,
     }
     print(json.dumps(line), flush=True)
+
+
+def batched_1g(args, w, np, host, n, base):
+    """The batched form on the resident input's first 1 GiB cut into 1,024
+    newline-aligned files of lognormal sizes: the filter-count once per file
+    (one launch each, two slots in flight) and over all of them in one launch
+    (warpjq_amd_submit_spans); both counts must equal one launch over the same
+    bytes."""
+    cut = min(n, 1 << 30)
+    while cut < n and host[cut - 1] != 10:
+        cut -= 1
+    sizes = file_sizes(np.random.default_rng(args.seed), 1024, cut, lo=4000, hi=64 << 20)
+    bounds, a = [], 0
+    for k, sz in enumerate(sizes):
+        if a >= cut:
+            break
+        b = cut if k == len(sizes) - 1 else min(cut, a + int(sz))
+        while b < cut and host[b - 1] != 10:
+            b += 1
+        bounds.append((a, b))
+        a = b
+    if a < cut:
+        bounds[-1] = (bounds[-1][0], cut)
+    ctx = w.Context(QUERY, chunk_bytes=64 << 20, n_slots=2)
+    try:
+        ctx.submit_device(0, base, cut)
+        want = ctx.wait(0).agg.count
+
+        def per_file():
+            count, pend = 0, None
+            for i, (a, b) in enumerate(bounds):
+                ctx.submit_device(i & 1, base + a, b - a)
+                if pend is not None:
+                    count += ctx.wait(pend).agg.count
+                pend = i & 1
+            return count + ctx.wait(pend).agg.count
+
+        def batched():
+            ctx.submit_spans(0, [(base + a, b - a) for a, b in bounds])
+            return ctx.wait(0).agg.count
+
+        res = {"files": len(bounds), "bytes": cut}
+        for name, fn in (("per_file", per_file), ("batched", batched)):
+            fn()
+            best = None
+            for _ in range(3):
+                t = time.perf_counter()
+                c = fn()
+                dt = time.perf_counter() - t
+                best = dt if best is None else min(best, dt)
+                if c != want:
+                    raise SystemExit(f"parity: {name} count {c} != {want} from one launch over the same bytes")
+            res[name + "_gbps"] = round(cut / best / 1e9, 1)
+        res["speedup"] = round(res["batched_gbps"] / res["per_file_gbps"], 2)
+        log("batched 1 GiB", res)
+        return res
+    finally:
+        ctx.close()
 
 
 CSV_QUERY = "select(.status >= 500) | {host: .host, status: .status, path: .path, bytes: .bytes}"
@@ -653,6 +713,7 @@ def main():
                          "kernel_ms": round(st["kernel_ms"], 1), "chunks": st["chunks"],
                          "h2d_gbps": round(n / (st["h2d_ms"] / 1e3) / 1e9, 1) if st["h2d_ms"] else None}
         log("e2e", extras["e2e"])
+        extras["batched_1g"] = batched_1g(args, w, np, host, n, base)
         # the nested workload's group_by over 1M users on a 1 GiB sample
         # (kernel chain; the full 4 GiB form is --workload nested)
         nh, nn, nlines, nbad = w.gen_nested(1 << 30, seed=args.seed, threads=min(16, os.cpu_count() or 1))
