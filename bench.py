@@ -7,7 +7,8 @@ the timed steps. Also reported, once each after the timed loop: the other
 three queries of the same workload (kernel-only GB/s), the end-to-end
 host-buffer run (pinned H2D + kernels), the batched form on the first 1 GiB
 cut into 1,024 files (one launch each against one launch for all), the
-roofline of the main kernel
+served form on the same 1 GiB as 8 resident datasets (p50/p99 at 1,000
+requests per second), the roofline of the main kernel
 (hipEvents around it, on its own stream) and the CPU baseline (the oracle on
 a bounded sample, host threads).
 
@@ -272,6 +273,70 @@ def batched_1g(args, w, np, host, n, base):
         return res
     finally:
         ctx.close()
+
+
+def served_1g(args, w, np, host, n):
+    """The served form on the input's first 1 GiB cut into 8 datasets of
+    log-uniform sizes, kept resident by a server (4 workers): 300 requests
+    from the 12 templates arriving as a Poisson process at 1,000 per second;
+    p50/p99 latency from arrival to result, and 12 of them again alone on the
+    idle server, whose bytes must be the same."""
+    import threading
+    rng = np.random.default_rng(args.seed + 1)
+    cut = min(n, 1 << 30)
+    while cut < n and host[cut - 1] != 10:
+        cut -= 1
+    w8 = np.exp(rng.uniform(np.log(1e6), np.log(5e8), size=8))
+    bounds, a = [], 0
+    for k, f in enumerate(w8 / w8.sum()):
+        b = cut if k == 7 else min(cut, a + int(f * cut))
+        while b < cut and host[b - 1] != 10:
+            b += 1
+        if b > a:
+            bounds.append((a, b))
+        a = b
+    srv = w.Server(hbm_budget=4 << 30, workers=4)
+    try:
+        ids = [srv.add_dataset(host[a:b]) for a, b in bounds]
+        for h in [srv.submit("count", [i]) for i in ids]:  # every dataset resident once
+            srv.wait(h)
+        reqs = [(served_query(rng), int(rng.integers(len(ids)))) for _ in range(300)]
+        gaps = rng.exponential(1.0 / 1000.0, size=len(reqs))
+        handles, done = [None] * len(reqs), [None] * len(reqs)
+
+        def collect():
+            for k in range(len(reqs)):
+                while handles[k] is None:
+                    time.sleep(0.0002)
+                out, _, lat = srv.wait(handles[k])
+                done[k] = (out, lat)
+
+        th = threading.Thread(target=collect)
+        th.start()
+        t0 = time.perf_counter()
+        nxt = t0
+        for k, (q, d) in enumerate(reqs):
+            nxt += gaps[k]
+            dt = nxt - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            handles[k] = srv.submit(q, [ids[d]])
+        th.join()
+        wall = time.perf_counter() - t0
+        for k in rng.choice(len(reqs), size=12, replace=False):
+            q, d = reqs[k]
+            alone, _, _ = srv.wait(srv.submit(q, [ids[d]]))
+            if alone != done[k][0]:
+                raise SystemExit(f"parity: served result differs alone vs in the mix: {q} on dataset {d}")
+        lats = np.array([x[1] for x in done])
+        nbytes = sum(bounds[d][1] - bounds[d][0] for _, d in reqs)
+        r = {"datasets": len(ids), "bytes": cut, "requests": len(reqs), "rate_per_s": 1000.0,
+             "p50_ms": round(float(np.percentile(lats, 50)), 3), "p99_ms": round(float(np.percentile(lats, 99)), 3),
+             "achieved_req_per_s": round(len(reqs) / wall, 1), "gbps": round(nbytes / wall / 1e9, 1)}
+        log("served 1 GiB", r)
+        return r
+    finally:
+        srv.close()
 
 
 CSV_QUERY = "select(.status >= 500) | {host: .host, status: .status, path: .path, bytes: .bytes}"
@@ -714,6 +779,7 @@ def main():
                          "h2d_gbps": round(n / (st["h2d_ms"] / 1e3) / 1e9, 1) if st["h2d_ms"] else None}
         log("e2e", extras["e2e"])
         extras["batched_1g"] = batched_1g(args, w, np, host, n, base)
+        extras["served_1g"] = served_1g(args, w, np, host, n)
         # the nested workload's group_by over 1M users on a 1 GiB sample
         # (kernel chain; the full 4 GiB form is --workload nested)
         nh, nn, nlines, nbad = w.gen_nested(1 << 30, seed=args.seed, threads=min(16, os.cpu_count() or 1))
