@@ -1324,8 +1324,12 @@ warpjq_status warpjq_wait(warpjq_ctx *ctx, uint32_t slot, warpjq_chunk_result *o
   DevCtr c = *s.h_ctr;
   if (!ctx->rows && !c.overflow && (c.first_bad != ~0ull || c.first_te_bad != ~0ull || c.fb_count) &&
       c.fb_count <= ctx->fb_cap) {
-    // aggregate queries recorded (claim, rank): turn them into line indices
-    hipLaunchKernelGGL(k_res_prefix, dim3(1), dim3(1024), 0, s.stream, s.d_tile_state, s.ntiles);
+    // aggregate queries recorded (claim, rank): turn them into line indices,
+    // the claims' line counts scanned as the row path scans them
+    const uint32_t nb = (s.ntiles + 1023) / 1024;
+    hipLaunchKernelGGL(k_tile_scan_blocks, dim3(nb), dim3(1024), 0, s.stream, s.d_tile_state, s.ntiles, s.d_tsum);
+    hipLaunchKernelGGL(k_tile_scan_totals, dim3(1), dim3(1024), 0, s.stream, s.d_tsum, nb);
+    hipLaunchKernelGGL(k_tile_scan_add, dim3(nb), dim3(1024), 0, s.stream, s.d_tile_state, s.ntiles, s.d_tsum);
     hipLaunchKernelGGL(k_res_records, dim3(std::max<uint32_t>(1, std::min<uint32_t>(1024, (c.fb_count + 255) / 256))),
                        dim3(256), 0, s.stream, s.d_ctr, s.d_tile_state, s.d_fb_idx, s.d_fb_tile, ctx->fb_cap);
     HIP_TRY(hipGetLastError(), "k_res");
